@@ -1,0 +1,170 @@
+/*
+ * taxila_lbm.h -- C ABI of the MI355X Shan-Chen flow step.
+ *
+ * This is the drop-in boundary for the flow timestep of Taxila LBM.  The
+ * Fortran-90 driver keeps its option names and its flow / distribution /
+ * walls objects; the bodies of the step procedures become calls into this
+ * library through ISO_C_BINDING (taxila-lbm_amd/fortran/txg_bindings.F90).
+ *
+ * Conventions shared by every entry point
+ *   - Every function returns int: TXG_OK (0) or a TXG_ERR_* code.  No C++
+ *     exception crosses the ABI.  txg_last_error() returns the message.
+ *   - Host arrays are borrowed only for the duration of a call and are laid
+ *     out like the reference's PETSc global vectors on one rank: owned nodes
+ *     in natural order (x fastest, then y, then z), DOF innermost:
+ *         walls[node]                       (double tags, 0 = pore)
+ *         rho  [node*C + m]                 (src/lbm/lbm_flow.F90:406-416)
+ *         flux [(node*D + d)*C + m]         (flux(1:C,1:D,node))
+ *         fi   [(node*Q + n)*C + m]         (fi(1:C,0:b,node), the fi%03d.dat layout)
+ *     with node = x + NX*(y + NY*z), C components, D dims, Q = b+1.
+ *   - One host thread drives a handle.  Each handle owns a HIP stream;
+ *     txg_step is stream-ordered and returns before the GPU finishes;
+ *     txg_get_*, txg_diagnostics, txg_reduce and txg_delta_norm synchronise.
+ */
+#ifndef TAXILA_LBM_H
+#define TAXILA_LBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TXG_MAX_COMPONENTS 4
+#define TXG_MAX_MINERALS 16
+
+/* return codes */
+#define TXG_OK 0
+#define TXG_ERR_ARG 1          /* invalid argument or descriptor */
+#define TXG_ERR_HIP 2          /* HIP runtime failure */
+#define TXG_ERR_STATE 3        /* call out of order (e.g. step before init) */
+#define TXG_ERR_UNSUPPORTED 4  /* option combination not implemented */
+#define TXG_ERR_NOMEM 5        /* device allocation failed */
+
+/* relaxation modes (include/lbm_definitions.h RELAXATION_MODE_*) */
+#define TXG_SRT 0
+#define TXG_MRT 1
+
+/* precision modes */
+#define TXG_FP64 0        /* FP64 compute and storage: parity mode */
+#define TXG_FP32 1        /* FP32 compute, FP32 population storage */
+#define TXG_FP32_FP16 2   /* FP32 compute, shifted FP16 storage (f_i - w_i rho0) */
+
+/* per-face boundary kinds (include/lbm_definitions.h BC_* values) */
+#define TXG_BC_NULL 0     /* periodic face (if periodic[d]) or ghost-wall bounce-back */
+#define TXG_BC_DENSITY 6  /* Dirichlet density, -bc_density_<face> (lbm_flow.F90:1066-1093) */
+
+/* faces, in the reference order BOUNDARY_XM..BOUNDARY_ZP (1..6) minus one */
+#define TXG_XM 0
+#define TXG_XP 1
+#define TXG_YM 2
+#define TXG_YP 3
+#define TXG_ZM 4
+#define TXG_ZP 5
+
+/*
+ * Run description: the options the reference reads in OptionsSetUp
+ * (src/lbm/lbm_options.F90:163-365), InfoSetFromOptions (lbm_info.F90:116-238),
+ * ComponentSetFromOptions (lbm_component.F90:131-176), RelaxationSetFromOptions
+ * (lbm_relaxation.F90:117-151), MineralSetFromOptions (lbm_mineral.F90:103-134)
+ * and FlowParseBC (lbm_flow.F90:1031-1309), as plain values.
+ * txg_desc_init() fills the reference defaults.
+ */
+typedef struct txg_desc {
+  int32_t ndims;            /* 2 = D2Q9, 3 = D3Q19 (-discretization) */
+  int32_t ncomponents;      /* -ncomponents, 1..TXG_MAX_COMPONENTS */
+  int32_t nx, ny, nz;       /* -NX -NY -NZ (nz = 1 for 2-D) */
+  int32_t periodic[3];      /* -bc_periodic_x/y/z */
+  int32_t isotropy_order;   /* -isotropy_order: 0, 4, 8 (and 10 for D2Q9) */
+  int32_t relaxation_mode;  /* -flow_relaxation_mode: TXG_SRT / TXG_MRT */
+  int32_t precision;        /* TXG_FP64 / TXG_FP32 / TXG_FP32_FP16 */
+  int32_t nminerals;        /* -nminerals, 0..TXG_MAX_MINERALS */
+  int32_t body_forces;      /* 1 if -gvt was given */
+  int32_t bc_flags[6];      /* TXG_BC_* per face, XM..ZP */
+  int32_t reserved;         /* keeps the doubles 8-byte aligned; set 0 */
+  double gvt[3];            /* -gvt */
+  double mm[TXG_MAX_COMPONENTS];    /* -mm_<name> */
+  double tau[TXG_MAX_COMPONENTS];   /* -tau_<name> (SRT) */
+  double s_c[TXG_MAX_COMPONENTS];   /* -s_c_<name> (MRT) */
+  double s_e[TXG_MAX_COMPONENTS];
+  double s_e2[TXG_MAX_COMPONENTS];
+  double s_q[TXG_MAX_COMPONENTS];
+  double s_nu[TXG_MAX_COMPONENTS];
+  double s_pi[TXG_MAX_COMPONENTS];
+  double s_m[TXG_MAX_COMPONENTS];
+  double g[TXG_MAX_COMPONENTS][TXG_MAX_COMPONENTS]; /* g[m][n] = -g_<m+1><n+1> */
+  double gw[TXG_MAX_MINERALS][TXG_MAX_COMPONENTS];  /* gw[mineral-1][m] = -gw_<mineral>_<name> */
+  double bc_density[6][TXG_MAX_COMPONENTS];          /* -bc_density_<face>_<name> */
+  double rho0[TXG_MAX_COMPONENTS];  /* FP16 storage shift density per component */
+} txg_desc;
+
+typedef struct txg_handle txg_handle;
+
+/* Fill d with the reference defaults: D3Q19, 1 component, SRT, tau = mm = 1,
+ * all MRT rates 1, isotropy 4, FP64, no walls, no forces
+ * (lbm_options.F90:93-152, lbm_relaxation.F90:58-80). */
+int txg_desc_init(txg_desc* d);
+
+/* Replaces LBMCreate + LBMSetFromOptions + LBMSetUp for the flow
+ * (lbm.F90:81-175; FlowSetUp lbm_flow.F90:395-430): validates d and
+ * allocates the device state for one domain on the current HIP device. */
+int txg_create(const txg_desc* d, txg_handle** h);
+
+/* Replaces WallsSetValues + WallsSetGhostNodes + WallsCommunicate
+ * (lbm_walls.F90:151-244): takes the owned wall tags (0 pore, 1..100 mineral
+ * id, 999 ghost wall) and builds the device node tags, the bounce-back link
+ * masks and the wettability link list. */
+int txg_set_walls(txg_handle* h, const double* walls);
+
+/* Replaces the initialize_state callback's output
+ * (default_initialize_state.F90:19): initial rho and u on owned nodes. */
+int txg_set_state(txg_handle* h, const double* rho, const double* u);
+
+/* Restart / IC-from-file path (lbm.F90:482-544): load populations directly;
+ * txg_init then skips FlowFiInit, as LBMInit2 does (lbm.F90:209-213). */
+int txg_set_fi(txg_handle* h, const double* fi);
+
+/* Replaces LBMInit2's flow part (lbm.F90:183-253): FlowFiInit (unless fi
+ * was loaded) and FlowUpdateMoments. */
+int txg_init(txg_handle* h);
+
+/* Replaces nsteps iterations of LBMRun2's flow body (lbm.F90:279-406):
+ * FlowCollision, DistributionCommunicateFi, FlowStream, FlowBounceback,
+ * FlowApplyBCs, FlowUpdateFlux.  Asynchronous on the handle's stream. */
+int txg_step(txg_handle* h, int32_t nsteps);
+
+/* Copies the current state back in the layouts above.  fi is the
+ * post-stream / post-BC population (what IOView(fi) writes); rho, flux (the
+ * common velocity u' of FlowUpdateUE) and forces are what the next
+ * collision uses.  Any pointer may be NULL. */
+int txg_get_fi(txg_handle* h, double* fi);
+int txg_get_moments(txg_handle* h, double* rho, double* flux, double* forces);
+
+/* FlowUpdateDiagnostics (lbm_flow.F90:603-708): total density, pressure
+ * (walls get null pressure 0) and total velocity on owned nodes.  Any
+ * pointer may be NULL.  velt[node*D + d]. */
+int txg_diagnostics(txg_handle* h, double* rhot, double* prs, double* velt);
+
+/* FP64 global sums over fluid nodes, reduced on the GPU:
+ * out[m] = sum rho_m, out[C + m*D + d] = sum (j + F/2)_{m,d}. */
+int txg_reduce(txg_handle* h, double* out);
+
+/* DistributionCalcDeltaNorm on rho (lbm_distribution_function.F90:809-833):
+ * max |(rho_old - rho)/rho| over owned node-components with rho != 0, then
+ * rho_old := rho.  The first call returns 1e99. */
+int txg_delta_norm(txg_handle* h, double* norm);
+
+/* Wait for all queued work on the handle's stream. */
+int txg_sync(txg_handle* h);
+
+/* Last error message of h (or of the last failed txg_create when h is
+ * NULL), NUL-terminated, truncated to len bytes.  Returns the length. */
+int txg_last_error(const txg_handle* h, char* buf, int32_t len);
+
+int txg_destroy(txg_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAXILA_LBM_H */

@@ -1,0 +1,158 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/taxila_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker and the CPU baseline.  The
+product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libtxg_oracle.so")
+SRC = os.path.join(HERE, "taxila_oracle.c")
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with gcc (-ffp-contract=off, OpenMP)."""
+    import subprocess
+    os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
+    inc = os.path.join(HERE, "..", "include")
+    cmd = ["gcc", "-O2", "-g", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off",
+           "-fno-fast-math", "-fopenmp", "-I", inc, "-I", HERE, "-o", LIB_PATH, SRC, "-lm"]
+    subprocess.run(cmd, check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        for name in ("orc_set_walls", "orc_set_fi", "orc_get_fi"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_init.argtypes = [C.c_void_p]
+        L.orc_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_moments.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_diagnostics.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_delta_norm.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_phase.argtypes = [C.c_void_p, C.c_int]
+        L.orc_ff_table.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 5 + [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+class Oracle:
+    """One domain of the FP64 restatement, driven like the reference."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        err = C.create_string_buffer(256)
+        self.h = lib().orc_create(C.byref(desc), err, 256)
+        if not self.h:
+            raise ValueError("oracle: " + err.value.decode())
+        self.C = desc.ncomponents
+        self.D = desc.ndims
+        self.Q = desc.Q
+        self.N = desc.nnodes
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().orc_destroy(h)
+            self.h = None
+
+    def set_walls(self, walls):
+        w = np.ascontiguousarray(walls, dtype=np.float64).ravel()
+        assert w.size == self.N
+        lib().orc_set_walls(self.h, _p(w))
+
+    def set_state(self, rho, u=None):
+        r = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+        assert r.size == self.N * self.C
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64).ravel()
+        lib().orc_set_state(self.h, _p(r), _p(uu))
+
+    def set_fi(self, fi):
+        f = np.ascontiguousarray(fi, dtype=np.float64).ravel()
+        assert f.size == self.N * self.Q * self.C
+        lib().orc_set_fi(self.h, _p(f))
+
+    def init(self):
+        lib().orc_init(self.h)
+
+    def step(self, n=1):
+        rc = lib().orc_step(self.h, int(n))
+        if rc:
+            raise RuntimeError("oracle step failed: %d" % rc)
+
+    def get_fi(self):
+        f = np.empty(self.N * self.Q * self.C)
+        lib().orc_get_fi(self.h, _p(f))
+        return f
+
+    def get_moments(self):
+        rho = np.empty(self.N * self.C)
+        flux = np.empty(self.N * self.C * self.D)
+        frc = np.empty(self.N * self.C * self.D)
+        lib().orc_get_moments(self.h, _p(rho), _p(flux), _p(frc))
+        return rho, flux, frc
+
+    def diagnostics(self):
+        rt = np.empty(self.N)
+        p = np.empty(self.N)
+        v = np.empty(self.N * self.D)
+        lib().orc_diagnostics(self.h, _p(rt), _p(p), _p(v))
+        return rt, p, v
+
+    def phase(self, *phases):
+        for ph in phases:
+            rc = lib().orc_phase(self.h, int(ph))
+            if rc:
+                raise RuntimeError('oracle phase %d failed' % ph)
+
+    def delta_norm(self):
+        out = np.zeros(1)
+        lib().orc_delta_norm(self.h, _p(out))
+        return float(out[0])
+
+
+def ff_table(dims: int, iso: int):
+    """The oracle's fluid-fluid stencil table: list of (offset, k, alternatives)."""
+    maxe = 96
+    off = np.zeros(3 * maxe, np.int32)
+    k = np.zeros(maxe, np.int32)
+    nalt = np.zeros(maxe, np.int32)
+    altlen = np.zeros(4 * maxe, np.int32)
+    cells = np.zeros(maxe * 4 * 2 * 3, np.int32)
+    n = lib().orc_ff_table(dims, iso, _p(off), _p(k), _p(nalt), _p(altlen), _p(cells), maxe)
+    out = []
+    for i in range(n):
+        alts = []
+        for a in range(nalt[i]):
+            cl = []
+            for c in range(altlen[4 * i + a]):
+                b = ((i * 4 + a) * 2 + c) * 3
+                cl.append(tuple(int(v) for v in cells[b:b + 3]))
+            alts.append(cl)
+        out.append((tuple(int(v) for v in off[3 * i:3 * i + 3]), int(k[i]), alts))
+    return out

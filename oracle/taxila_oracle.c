@@ -1,0 +1,1190 @@
+/*
+ * taxila_oracle.c -- CPU restatement of the Taxila LBM flow step (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU
+ * baseline of bench.py.  Nothing in the product links or calls this file.
+ *
+ * Scope: D2Q9 / D3Q19, SRT / MRT, fluid-fluid forces at isotropy 4/8 (and
+ * 10 on D2Q9), per-mineral fluid-solid forces, body forces, bounce-back
+ * (including ghost-999 faces and 900-902 specular tags), periodic faces and
+ * Dirichlet density faces.  psi = rho (no -flow_use_nonideal_eos).
+ *
+ * Arrays follow the reference's local (ghosted) PETSc arrays, 1-based owned
+ * coordinates, component index fastest:
+ *   fi(1:C, 0:b, gxs:gxe, gys:gye, gzs:gze)     ghost width 1
+ *   rho(1:C, rgxs:rgxe, ...), walls(rgxs:rgxe, ...) ghost width 1/2/3
+ *   flux, forces(1:C, 1:D, gxs:gxe, ...)
+ * Expressions are written in the reference's evaluation order, including
+ * the single-precision literals of the fluid-solid force; build with
+ * -ffp-contract=off so that no multiply-add is fused.
+ *
+ * Parity status of each part (how the restatement is pinned):
+ *   - D2Q9 SRT iso-4 and D2Q9 MRT iso-10, periodic: the reference's golden
+ *     fixtures tests/bubble_2D{,_hots}/reference_solution/fi001.dat.
+ *   - D3Q19 equilibrium, MRT, iso-8 stencil, fluid-solid and body forces,
+ *     bounce-back, Dirichlet faces: closed-form and invariant checks
+ *     (moments of feq, MRT with equal rates == SRT, mass/momentum
+ *     conservation, linear-field gradient, hand-computed wall forces,
+ *     symmetry/rotation equivariance) and the stencil table extracted from
+ *     the reference text.  No reference output pins D3Q19 numerically.
+ *   - Steady-state delta norm at rho == 0 nodes: parity unpinned (PETSc
+ *     VecPointwiseDivide semantics at zero divisors; zeros are skipped here).
+ */
+#include "taxila_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXC TXG_MAX_COMPONENTS
+
+/* include/lbm_definitions.h:105-111 */
+#define WALL_GHOST 999.0
+#define WALL_NORMAL_X 900.0
+#define WALL_NORMAL_Y 901.0
+#define WALL_NORMAL_Z 902.0
+
+struct orc {
+  txg_desc d;
+  int D, Q, b, C;
+  int NX, NY, NZ;
+  int wr; /* rho / walls ghost width (lbm_grid.F90:107-120) */
+  /* f-ghost box in 1-based coordinates */
+  int gxs, gys, gzs, gxl, gyl, gzl;
+  /* rho-ghost box */
+  int rgxs, rgys, rgzs, rgxl, rgyl, rgzl;
+  long gN, rgN;
+  double *fi, *tmp, *fi_eq, *rho, *walls, *flux, *forces, *rho_old;
+  int have_old;
+  /* lattice (lbm_discretization_d3q19.F90:64-232, _d2q9.F90:53-144) */
+  int ci[19][3];
+  double w[19];
+  int opp[19], refl[3][19];
+  double mt[19][19]; /* mt[n][k] = mt_mrt(k, n): moment n, direction k */
+  double mmt[19];
+  double ffw[16];
+  double c0;
+  /* components (lbm_component.F90:131-176, lbm_relaxation.F90:117-151) */
+  double d_k[MAXC], c_s2[MAXC], tau[MAXC], s_c[MAXC], tau_mrt[MAXC][19];
+  int ff, fs, body;
+  int dirichlet[6];
+  int fi_loaded, initialized;
+  /* fluid-fluid stencil */
+  int nff;
+  int ffoff[96][3], ffk[96], ffnalt[96], ffaltlen[96][4], ffalt[96][4][2][3];
+};
+
+/* ---------------------------------------------------------------- indexing */
+static inline long FIX(const orc_t* o, int x, int y, int z) {
+  return ((long)(z - o->gzs) * o->gyl + (y - o->gys)) * o->gxl + (x - o->gxs);
+}
+static inline long RIX(const orc_t* o, int x, int y, int z) {
+  return ((long)(z - o->rgzs) * o->rgyl + (y - o->rgys)) * o->rgxl + (x - o->rgxs);
+}
+#define FI(o, x, y, z, n, m) (o)->fi[(FIX(o, x, y, z) * (o)->Q + (n)) * (o)->C + (m)]
+#define FEQ(o, x, y, z, n, m) (o)->fi_eq[(FIX(o, x, y, z) * (o)->Q + (n)) * (o)->C + (m)]
+#define RHO(o, x, y, z, m) (o)->rho[RIX(o, x, y, z) * (o)->C + (m)]
+#define WALL(o, x, y, z) (o)->walls[RIX(o, x, y, z)]
+#define FLUX(o, x, y, z, d, m) (o)->flux[(FIX(o, x, y, z) * (o)->D + (d)) * (o)->C + (m)]
+#define FRC(o, x, y, z, d, m) (o)->forces[(FIX(o, x, y, z) * (o)->D + (d)) * (o)->C + (m)]
+
+static int g_threads = 0;
+int orc_set_threads(int n) {
+  g_threads = n;
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#endif
+  return 0;
+}
+
+/* ------------------------------------------------------------------ lattice */
+static void setup_d3q19(orc_t* o) {
+  /* lbm_discretization_d3q19.F90:153-197 */
+  static const int cx[19] = {0, 1, 0, -1, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0};
+  static const int cy[19] = {0, 0, 1, 0, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, -1, 1};
+  static const int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+  static const int M[19][19] = {
+      {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {-30, -11, -11, -11, -11, -11, -11, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8},
+      {12, -4, -4, -4, -4, -4, -4, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {0, 1, 0, -1, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0},
+      {0, -4, 0, 4, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0},
+      {0, 0, 1, 0, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, -1, 1},
+      {0, 0, -4, 0, 4, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, -1, 1},
+      {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1},
+      {0, 0, 0, 0, 0, -4, 4, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1},
+      {0, 2, -1, 2, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+      {0, -4, 2, -4, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+      {0, 0, 1, 0, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+      {0, 0, -2, 0, -2, 2, 2, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, 1, -1, -1, 1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, -1, -1, 1, 1}};
+  static const int mmt[19] = {19, 2394, 252, 10, 40, 10, 40, 10, 40, 36, 72, 12, 24, 4, 4, 4, 8, 8, 8};
+  o->D = 3;
+  o->b = 18;
+  o->Q = 19;
+  for (int n = 0; n < 19; n++) {
+    o->ci[n][0] = cx[n];
+    o->ci[n][1] = cy[n];
+    o->ci[n][2] = cz[n];
+    o->mmt[n] = (double)mmt[n];
+    for (int k = 0; k < 19; k++) o->mt[n][k] = (double)M[n][k];
+  }
+  o->w[0] = 1.0 / 3.0;
+  for (int n = 1; n <= 6; n++) o->w[n] = 1.0 / 18.0;
+  for (int n = 7; n <= 18; n++) o->w[n] = 1.0 / 36.0;
+  o->c0 = 6.0;
+  /* ffw (lbm_discretization_d3q19.F90:199-230); d1/3. style literals are
+   * double/real(exact integer) = correctly rounded double quotients */
+  memset(o->ffw, 0, sizeof(o->ffw));
+  if (o->d.isotropy_order == 4) {
+    o->ffw[1] = 1.0 / 6.0;
+    o->ffw[2] = 1.0 / 12.0;
+  } else if (o->d.isotropy_order == 8) {
+    o->ffw[1] = 4.0 / 45.0;
+    o->ffw[2] = 1.0 / 21.0;
+    o->ffw[3] = 2.0 / 105.0;
+    o->ffw[4] = 5.0 / 504.0;
+    o->ffw[5] = 1.0 / 315.0;
+    o->ffw[6] = 1.0 / 630.0;
+    o->ffw[8] = 1.0 / 5040.0;
+  }
+}
+
+static void setup_d2q9(orc_t* o) {
+  /* lbm_discretization_d2q9.F90:99-142 */
+  static const int cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+  static const int cy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+  static const int M[9][9] = {{1, 1, 1, 1, 1, 1, 1, 1, 1},      {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+                              {4, -2, -2, -2, -2, 1, 1, 1, 1},  {0, 1, 0, -1, 0, 1, -1, -1, 1},
+                              {0, -2, 0, 2, 0, 1, -1, -1, 1},   {0, 0, 1, 0, -1, 1, 1, -1, -1},
+                              {0, 0, -2, 0, 2, 1, 1, -1, -1},   {0, 1, -1, 1, -1, 0, 0, 0, 0},
+                              {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+  static const int mmt[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
+  o->D = 2;
+  o->b = 8;
+  o->Q = 9;
+  for (int n = 0; n < 9; n++) {
+    o->ci[n][0] = cx[n];
+    o->ci[n][1] = cy[n];
+    o->ci[n][2] = 0;
+    o->mmt[n] = (double)mmt[n];
+    for (int k = 0; k < 9; k++) o->mt[n][k] = (double)M[n][k];
+  }
+  o->w[0] = 4.0 / 9.0;
+  for (int n = 1; n <= 4; n++) o->w[n] = 1.0 / 9.0;
+  for (int n = 5; n <= 8; n++) o->w[n] = 1.0 / 36.0;
+  o->c0 = 6.0;
+  memset(o->ffw, 0, sizeof(o->ffw));
+  if (o->d.isotropy_order == 4) {
+    o->ffw[1] = 1.0 / 3.0;
+    o->ffw[2] = 1.0 / 12.0;
+  } else if (o->d.isotropy_order == 8) {
+    o->ffw[1] = 4.0 / 21.0;
+    o->ffw[2] = 4.0 / 45.0;
+    o->ffw[4] = 1.0 / 60.0;
+    o->ffw[5] = 2.0 / 315.0;
+    o->ffw[8] = 1.0 / 5040.0;
+  } else if (o->d.isotropy_order == 10) {
+    o->ffw[1] = 262.0 / 1785.0;
+    o->ffw[2] = 93.0 / 1190.0;
+    o->ffw[4] = 7.0 / 340.0;
+    o->ffw[5] = 6.0 / 595.0;
+    o->ffw[8] = 9.0 / 9520.0;
+    o->ffw[9] = 2.0 / 5355.0;
+    o->ffw[10] = 1.0 / 7140.0;
+  }
+}
+
+/* opposite and reflected directions: the direction whose velocity is the
+ * negated / mirrored one (lbm_discretization_d3q19.F90:80-150) */
+static void setup_maps(orc_t* o) {
+  for (int n = 0; n < o->Q; n++) {
+    for (int k = 0; k < o->Q; k++) {
+      int* a = o->ci[n];
+      int* c = o->ci[k];
+      if (c[0] == -a[0] && c[1] == -a[1] && c[2] == -a[2]) o->opp[n] = k;
+      if (c[0] == -a[0] && c[1] == a[1] && c[2] == a[2]) o->refl[0][n] = k;
+      if (c[0] == a[0] && c[1] == -a[1] && c[2] == a[2]) o->refl[1][n] = k;
+      if (c[0] == a[0] && c[1] == a[1] && c[2] == -a[2]) o->refl[2][n] = k;
+    }
+  }
+}
+
+/* ------------------------------------------------- fluid-fluid stencil table
+ * Offsets in the order the reference accumulates them
+ * (lbm_forcing.F90:84-939 for D3, :985-1285 for D2).  For each offset e the
+ * weight index is |e|^2, the gradient term is sign(e_d)*|e_d|*ffw*(drho) and
+ * the weight sum term ffw*e_d^2 (checked against the reference text by the
+ * stencil fixture in tests/golden).  The wall test is: target fluid AND (no
+ * alternatives, or all cells of at least one alternative fluid). */
+static const int FF3[92][3] = {
+    /* |e|^2 = 1, 2 (isotropy 4) */
+    {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+    {1, 1, 0}, {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1}, {-1, 0, 1},
+    {1, 0, -1}, {-1, 0, -1}, {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1},
+    /* |e|^2 = 3 */
+    {1, 1, 1}, {-1, 1, 1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, -1}, {1, -1, -1},
+    /* |e|^2 = 4 */
+    {2, 0, 0}, {-2, 0, 0}, {0, 2, 0}, {0, -2, 0}, {0, 0, 2}, {0, 0, -2},
+    /* |e|^2 = 5 */
+    {2, 1, 0}, {-2, 1, 0}, {-2, -1, 0}, {2, -1, 0}, {2, 0, 1}, {-2, 0, 1}, {-2, 0, -1}, {2, 0, -1},
+    {1, 2, 0}, {-1, 2, 0}, {-1, -2, 0}, {1, -2, 0}, {0, 2, 1}, {0, -2, 1}, {0, 2, -1}, {0, -2, -1},
+    {1, 0, 2}, {0, 1, 2}, {-1, 0, 2}, {0, -1, 2}, {1, 0, -2}, {0, 1, -2}, {-1, 0, -2}, {0, -1, -2},
+    /* |e|^2 = 6 */
+    {2, 1, 1}, {-2, 1, 1}, {-2, -1, 1}, {2, -1, 1}, {2, 1, -1}, {-2, 1, -1}, {-2, -1, -1}, {2, -1, -1},
+    {1, 2, 1}, {-1, 2, 1}, {-1, -2, 1}, {1, -2, 1}, {1, 2, -1}, {-1, -2, -1}, {-1, 2, -1}, {1, -2, -1},
+    {1, 1, 2}, {-1, 1, 2}, {-1, -1, 2}, {1, -1, 2}, {1, 1, -2}, {-1, 1, -2}, {-1, -1, -2}, {1, -1, -2},
+    /* |e|^2 = 8 */
+    {2, 2, 0}, {-2, 2, 0}, {-2, -2, 0}, {2, -2, 0}, {2, 0, 2}, {-2, 0, 2}, {-2, 0, -2}, {2, 0, -2},
+    {0, 2, 2}, {0, -2, 2}, {0, -2, -2}, {0, 2, -2}};
+
+static const int FF2[36][2] = {
+    {1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1},
+    {2, 0}, {-2, 0}, {0, 2}, {0, -2},
+    {2, 1}, {2, -1}, {-2, 1}, {-2, -1}, {1, 2}, {1, -2}, {-1, 2}, {-1, -2},
+    {2, 2}, {-2, 2}, {2, -2}, {-2, -2},
+    {3, 0}, {-3, 0}, {0, 3}, {0, -3},
+    {3, 1}, {3, -1}, {-3, 1}, {-3, -1}, {1, 3}, {1, -3}, {-1, 3}, {-1, -3}};
+
+static int isgn(int v) { return (v > 0) - (v < 0); }
+
+/* wall-test alternatives of offset e, by the rule the reference's
+ * conditions follow (lbm_forcing.F90:314-939, :1076-1285) */
+static void ff_conditions(const int e[3], int* nalt, int altlen[4], int alt[4][2][3]) {
+  int a[3] = {abs(e[0]), abs(e[1]), abs(e[2])};
+  int L = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+  int mx = a[0] > a[1] ? (a[0] > a[2] ? 0 : 2) : (a[1] > a[2] ? 1 : 2); /* axis of largest |e_d| */
+  *nalt = 0;
+  if (L <= 3) return;
+  if (L == 4 || L == 8) { /* midpoint */
+    *nalt = 1;
+    altlen[0] = 1;
+    for (int d = 0; d < 3; d++) alt[0][0][d] = e[d] / 2;
+    return;
+  }
+  if (L == 5 || L == 6) { /* step the 2-axis by one, the 1-axes by 0 or 1 */
+    int h[3];
+    for (int d = 0; d < 3; d++) h[d] = (d == mx) ? isgn(e[d]) : e[d];
+    int ones[2], no = 0;
+    for (int d = 0; d < 3; d++)
+      if (d != mx && e[d] != 0) ones[no++] = d;
+    int cnt = 1 << no;
+    for (int s = 0; s < cnt; s++) {
+      altlen[*nalt] = 1;
+      for (int d = 0; d < 3; d++) alt[*nalt][0][d] = h[d];
+      for (int q = 0; q < no; q++)
+        if (s & (1 << q)) alt[*nalt][0][ones[q]] = 0;
+      (*nalt)++;
+    }
+    return;
+  }
+  if (L == 9) { /* both cells on the axis */
+    *nalt = 1;
+    altlen[0] = 2;
+    for (int d = 0; d < 3; d++) {
+      alt[0][0][d] = isgn(e[d]);
+      alt[0][1][d] = 2 * isgn(e[d]);
+    }
+    return;
+  }
+  if (L == 10) { /* both cells of the straight path, on either row */
+    *nalt = 2;
+    for (int r = 0; r < 2; r++) {
+      altlen[r] = 2;
+      for (int d = 0; d < 3; d++) {
+        int base = (d == mx) ? isgn(e[d]) : (r ? e[d] : 0);
+        int base2 = (d == mx) ? 2 * isgn(e[d]) : (r ? e[d] : 0);
+        alt[r][0][d] = base;
+        alt[r][1][d] = base2;
+      }
+    }
+    return;
+  }
+}
+
+static int ff_count(int dims, int iso) {
+  if (dims == 3) return iso == 4 ? 18 : (iso == 8 ? 92 : 0);
+  return iso == 4 ? 8 : (iso == 8 ? 24 : (iso == 10 ? 36 : 0));
+}
+
+static void build_ff(orc_t* o) {
+  int nent = ff_count(o->D, o->d.isotropy_order);
+  o->nff = nent;
+  for (int i = 0; i < nent; i++) {
+    int e[3] = {0, 0, 0};
+    if (o->D == 3) {
+      e[0] = FF3[i][0];
+      e[1] = FF3[i][1];
+      e[2] = FF3[i][2];
+    } else {
+      e[0] = FF2[i][0];
+      e[1] = FF2[i][1];
+    }
+    memcpy(o->ffoff[i], e, sizeof(e));
+    o->ffk[i] = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+    ff_conditions(e, &o->ffnalt[i], o->ffaltlen[i], o->ffalt[i]);
+  }
+}
+
+int orc_ff_table(int dims, int iso, int* off, int* kidx, int* nalt, int* altlen, int* altcells, int maxent) {
+  int nent = ff_count(dims, iso);
+  if (nent > maxent) return -1;
+  for (int i = 0; i < nent; i++) {
+    int e[3] = {0, 0, 0};
+    if (dims == 3) {
+      e[0] = FF3[i][0];
+      e[1] = FF3[i][1];
+      e[2] = FF3[i][2];
+    } else {
+      e[0] = FF2[i][0];
+      e[1] = FF2[i][1];
+    }
+    int na, al[4] = {0, 0, 0, 0}, ac[4][2][3];
+    memset(ac, 0, sizeof(ac));
+    ff_conditions(e, &na, al, ac);
+    for (int d = 0; d < 3; d++) off[3 * i + d] = e[d];
+    kidx[i] = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+    nalt[i] = na;
+    for (int a = 0; a < 4; a++) {
+      altlen[4 * i + a] = al[a];
+      for (int c = 0; c < 2; c++)
+        for (int d = 0; d < 3; d++) altcells[((i * 4 + a) * 2 + c) * 3 + d] = ac[a][c][d];
+    }
+  }
+  return nent;
+}
+
+/* ------------------------------------------------------------------ create */
+orc_t* orc_create(const txg_desc* d, char* err, int errlen) {
+#define FAIL(msg)                                  \
+  do {                                             \
+    if (err) snprintf(err, (size_t)errlen, "%s", msg); \
+    free(o);                                       \
+    return NULL;                                   \
+  } while (0)
+  orc_t* o = (orc_t*)calloc(1, sizeof(orc_t));
+  if (!o) return NULL;
+  o->d = *d;
+  if (d->ndims != 2 && d->ndims != 3) FAIL("ndims must be 2 or 3");
+  if (d->ncomponents < 1 || d->ncomponents > MAXC) FAIL("bad ncomponents");
+  if (d->nx < 3 || d->ny < 3 || (d->ndims == 3 && d->nz < 3)) FAIL("domain too small");
+  if (d->nminerals < 0 || d->nminerals > TXG_MAX_MINERALS) FAIL("bad nminerals");
+  int iso = d->isotropy_order;
+  if (!(iso == 0 || iso == 4 || iso == 8 || (iso == 10 && d->ndims == 2))) FAIL("bad isotropy_order");
+  o->C = d->ncomponents;
+  if (d->ndims == 3)
+    setup_d3q19(o);
+  else
+    setup_d2q9(o);
+  setup_maps(o);
+  build_ff(o);
+  o->NX = d->nx;
+  o->NY = d->ny;
+  o->NZ = d->ndims == 3 ? d->nz : 1;
+  /* lbm_grid.F90:107-120; width 1 at isotropy 0 keeps walls addressable by
+   * the width-1 bounce-back scan */
+  o->wr = iso == 8 ? 2 : (iso == 10 ? 3 : 1);
+  o->gxs = 0;
+  o->gys = 0;
+  o->gxl = o->NX + 2;
+  o->gyl = o->NY + 2;
+  o->rgxs = 1 - o->wr;
+  o->rgys = 1 - o->wr;
+  o->rgxl = o->NX + 2 * o->wr;
+  o->rgyl = o->NY + 2 * o->wr;
+  if (o->D == 3) {
+    o->gzs = 0;
+    o->gzl = o->NZ + 2;
+    o->rgzs = 1 - o->wr;
+    o->rgzl = o->NZ + 2 * o->wr;
+  } else {
+    o->gzs = 1;
+    o->gzl = 1;
+    o->rgzs = 1;
+    o->rgzl = 1;
+  }
+  o->gN = (long)o->gxl * o->gyl * o->gzl;
+  o->rgN = (long)o->rgxl * o->rgyl * o->rgzl;
+  long nf = o->gN * o->Q * o->C;
+  o->fi = (double*)calloc((size_t)nf, sizeof(double));
+  o->tmp = (double*)calloc((size_t)nf, sizeof(double));
+  o->fi_eq = (double*)calloc((size_t)nf, sizeof(double));
+  o->rho = (double*)calloc((size_t)(o->rgN * o->C), sizeof(double));
+  o->rho_old = (double*)calloc((size_t)((long)o->NX * o->NY * o->NZ * o->C), sizeof(double));
+  o->walls = (double*)calloc((size_t)o->rgN, sizeof(double));
+  o->flux = (double*)calloc((size_t)(o->gN * o->D * o->C), sizeof(double));
+  o->forces = (double*)calloc((size_t)(o->gN * o->D * o->C), sizeof(double));
+  if (!o->fi || !o->tmp || !o->fi_eq || !o->rho || !o->rho_old || !o->walls || !o->flux || !o->forces) {
+    orc_destroy(o);
+    if (err) snprintf(err, (size_t)errlen, "out of memory");
+    return NULL;
+  }
+  for (int m = 0; m < o->C; m++) {
+    /* lbm_relaxation.F90:79 c_s2 = 1.d0/3.; lbm_component.F90:158 d_k */
+    o->c_s2[m] = 1.0 / 3.0;
+    o->d_k[m] = 1. - 2. / (3. * d->mm[m]);
+    if (d->relaxation_mode == TXG_SRT) {
+      o->tau[m] = d->tau[m];
+      o->s_c[m] = 1.0 / d->tau[m]; /* lbm_relaxation.F90:133 */
+    } else {
+      o->tau[m] = 1.0; /* MRT never reads -tau (lbm_relaxation.F90:128-150) */
+      o->s_c[m] = d->s_c[m];
+    }
+    /* rate maps, lbm_discretization_d3q19.F90:234-263, _d2q9.F90:147-166 */
+    double* t = o->tau_mrt[m];
+    if (o->D == 3) {
+      double r[19] = {d->s_c[m],  d->s_e[m],  d->s_e2[m], d->s_c[m],  d->s_q[m], d->s_c[m],  d->s_q[m],
+                      d->s_c[m],  d->s_q[m],  d->s_nu[m], d->s_pi[m], d->s_nu[m], d->s_pi[m], d->s_nu[m],
+                      d->s_nu[m], d->s_nu[m], d->s_m[m],  d->s_m[m],  d->s_m[m]};
+      memcpy(t, r, sizeof(r));
+    } else {
+      double r[9] = {d->s_c[m], d->s_e[m], d->s_e2[m], d->s_c[m], d->s_q[m],
+                     d->s_c[m], d->s_q[m], d->s_nu[m], d->s_nu[m]};
+      memcpy(t, r, sizeof(r));
+    }
+  }
+  /* lbm_flow.F90:209-215, lbm_walls.F90:117-125: eps = 1.e-15 (a default
+   * real literal) */
+  const double eps15 = (double)1.e-15f;
+  o->ff = 0;
+  for (int m = 0; m < o->C; m++)
+    for (int n = 0; n < o->C; n++)
+      if (fabs(d->g[m][n]) > eps15) o->ff = 1;
+  o->fs = 0;
+  for (int k = 0; k < d->nminerals; k++)
+    for (int m = 0; m < o->C; m++)
+      if (fabs(d->gw[k][m]) > eps15) o->fs = 1;
+  o->body = d->body_forces != 0;
+  for (int f = 0; f < 6; f++) {
+    int dim = f / 2;
+    if (dim >= o->D) continue;
+    if (d->bc_flags[f] == TXG_BC_DENSITY) {
+      if (d->periodic[dim]) FAIL("density BC on a periodic face");
+      o->dirichlet[f] = 1;
+    } else if (d->bc_flags[f] != TXG_BC_NULL) {
+      FAIL("unsupported boundary condition kind");
+    }
+  }
+  return o;
+#undef FAIL
+}
+
+void orc_destroy(orc_t* o) {
+  if (!o) return;
+  free(o->fi);
+  free(o->tmp);
+  free(o->fi_eq);
+  free(o->rho);
+  free(o->rho_old);
+  free(o->walls);
+  free(o->flux);
+  free(o->forces);
+  free(o);
+}
+
+/* ---------------------------------------------------------- halo exchange
+ * DMLocalToLocal on one rank: a ghost point is refreshed from its periodic
+ * image when every out-of-range coordinate lies on a periodic axis; ghost
+ * points beyond a non-periodic (DM_BOUNDARY_GHOSTED) face are left alone
+ * (lbm_grid.F90:151-156, lbm_distribution_function.F90:309-357). */
+static void halo(orc_t* o, double* a, int dof, int rbox) {
+  int xs = rbox ? o->rgxs : o->gxs, ys = rbox ? o->rgys : o->gys, zs = rbox ? o->rgzs : o->gzs;
+  int xl = rbox ? o->rgxl : o->gxl, yl = rbox ? o->rgyl : o->gyl, zl = rbox ? o->rgzl : o->gzl;
+  int N[3] = {o->NX, o->NY, o->NZ};
+  for (int z = zs; z < zs + zl; z++)
+    for (int y = ys; y < ys + yl; y++)
+      for (int x = xs; x < xs + xl; x++) {
+        int c[3] = {x, y, z}, img[3];
+        int ghost = 0, ok = 1;
+        for (int d = 0; d < 3; d++) {
+          img[d] = c[d];
+          if (d >= o->D) continue;
+          if (c[d] < 1 || c[d] > N[d]) {
+            ghost = 1;
+            if (!o->d.periodic[d]) {
+              ok = 0;
+            } else {
+              img[d] = ((c[d] - 1) % N[d] + N[d]) % N[d] + 1;
+            }
+          }
+        }
+        if (!ghost || !ok) continue;
+        long dst = ((long)(z - zs) * yl + (y - ys)) * xl + (x - xs);
+        long src = ((long)(img[2] - zs) * yl + (img[1] - ys)) * xl + (img[0] - xs);
+        memcpy(a + dst * dof, a + src * dof, sizeof(double) * (size_t)dof);
+      }
+}
+
+/* ------------------------------------------------------------ set-up input */
+int orc_set_walls(orc_t* o, const double* walls) {
+  memset(o->walls, 0, sizeof(double) * (size_t)o->rgN);
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++)
+        WALL(o, x, y, z) = walls[(x - 1) + (long)o->NX * ((y - 1) + (long)o->NY * (z - 1))];
+  /* WallsSetGhostNodesD2/D3 (lbm_walls.F90:185-231) */
+  int N[3] = {o->NX, o->NY, o->NZ};
+  int lo[3] = {o->rgxs, o->rgys, o->rgzs}, len[3] = {o->rgxl, o->rgyl, o->rgzl};
+  for (int dim = 0; dim < o->D; dim++) {
+    if (o->d.periodic[dim]) continue;
+    for (int z = lo[2]; z < lo[2] + len[2]; z++)
+      for (int y = lo[1]; y < lo[1] + len[1]; y++)
+        for (int x = lo[0]; x < lo[0] + len[0]; x++) {
+          int c[3] = {x, y, z};
+          if (c[dim] < 1 || c[dim] > N[dim]) WALL(o, x, y, z) = WALL_GHOST;
+        }
+  }
+  /* WallsCommunicate (lbm_walls.F90:233-244) */
+  halo(o, o->walls, 1, 1);
+  return 0;
+}
+
+int orc_set_state(orc_t* o, const double* rho, const double* u) {
+  /* initialize_state fills owned rho and u (= flux) */
+  memset(o->rho, 0, sizeof(double) * (size_t)(o->rgN * o->C));
+  memset(o->flux, 0, sizeof(double) * (size_t)(o->gN * o->D * o->C));
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        long node = (x - 1) + (long)o->NX * ((y - 1) + (long)o->NY * (z - 1));
+        for (int m = 0; m < o->C; m++) RHO(o, x, y, z, m) = rho[node * o->C + m];
+        if (u)
+          for (int d = 0; d < o->D; d++)
+            for (int m = 0; m < o->C; m++) FLUX(o, x, y, z, d, m) = u[(node * o->D + d) * o->C + m];
+      }
+  o->fi_loaded = 0;
+  return 0;
+}
+
+int orc_set_fi(orc_t* o, const double* fi) {
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        long node = (x - 1) + (long)o->NX * ((y - 1) + (long)o->NY * (z - 1));
+        for (int n = 0; n < o->Q; n++)
+          for (int m = 0; m < o->C; m++) FI(o, x, y, z, n, m) = fi[(node * o->Q + n) * o->C + m];
+      }
+  o->fi_loaded = 1;
+  return 0;
+}
+
+/* ------------------------------------------------------------ moments */
+/* DistributionCalcDensityD2/D3 (lbm_distribution_function.F90:374-428) */
+static void calc_density(orc_t* o) {
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        if (WALL(o, x, y, z) == 0) {
+          for (int m = 0; m < o->C; m++) {
+            double s = 0.0;
+            for (int n = 0; n < o->Q; n++) s = s + FI(o, x, y, z, n, m);
+            RHO(o, x, y, z, m) = s;
+          }
+        } else {
+          for (int m = 0; m < o->C; m++) RHO(o, x, y, z, m) = 0.;
+        }
+      }
+}
+
+/* DistributionCalcFluxD2/D3 (lbm_distribution_function.F90:451-515) */
+static void calc_flux_into(orc_t* o, double* u) {
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        long base = FIX(o, x, y, z) * o->D * o->C;
+        if (WALL(o, x, y, z) == 0) {
+          for (int m = 0; m < o->C; m++)
+            for (int d = 0; d < o->D; d++) {
+              double s = 0.0;
+              for (int n = 0; n < o->Q; n++) s = s + FI(o, x, y, z, n, m) * (double)o->ci[n][d];
+              u[base + d * o->C + m] = s;
+            }
+        } else {
+          for (int k = 0; k < o->D * o->C; k++) u[base + k] = 0.;
+        }
+      }
+}
+
+/* ------------------------------------------------------------ forces */
+/* LBMAddFluidSolidForcesD3/D2 (lbm_forcing.F90:1326-1421) with
+ * DistributionGatherValueToDirection (lbm_distribution_function.F90:786-807).
+ * 1./6., 1./12. (D3) and 1./3., 1./12. (D2) are default-real literals. */
+static void add_fluid_solid(orc_t* o) {
+  const double kax = o->D == 3 ? (double)(1.f / 6.f) : (double)(1.f / 3.f);
+  const double kdg = (double)(1.f / 12.f);
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        if (WALL(o, x, y, z) != 0) continue;
+        for (int n = 1; n <= o->b; n++) {
+          double t = WALL(o, x + o->ci[n][0], y + o->ci[n][1], z + o->ci[n][2]);
+          if (t > 0. && t < 998.) {
+            int mineral = (int)t;
+            double k = (n <= 2 * o->D) ? kax : kdg;
+            for (int d = 0; d < o->D; d++)
+              for (int m = 0; m < o->C; m++) {
+                double gw = (mineral >= 1 && mineral <= o->d.nminerals) ? o->d.gw[mineral - 1][m] : 0.0;
+                FRC(o, x, y, z, d, m) =
+                    FRC(o, x, y, z, d, m) - k * RHO(o, x, y, z, m) * gw * (double)o->ci[n][d];
+              }
+          }
+        }
+      }
+}
+
+/* LBMAddBodyForcesD3/D2 (lbm_forcing.F90:1440-1496) */
+static void add_body(orc_t* o) {
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        if (WALL(o, x, y, z) != 0) continue;
+        for (int m = 0; m < o->C; m++)
+          for (int d = 0; d < o->D; d++)
+            FRC(o, x, y, z, d, m) = FRC(o, x, y, z, d, m) + o->d.gvt[d] * o->d.mm[m] * RHO(o, x, y, z, m);
+      }
+}
+
+/* LBMAddFluidFluidForcesD3/D2 (lbm_forcing.F90:51-958, :960-1299) */
+static void add_fluid_fluid(orc_t* o) {
+  const double eps12 = (double)1.e-12f;
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        if (WALL(o, x, y, z) != 0) continue;
+        double grad[MAXC][3], ws[3] = {0., 0., 0.};
+        for (int m = 0; m < o->C; m++) grad[m][0] = grad[m][1] = grad[m][2] = 0.;
+        for (int i = 0; i < o->nff; i++) {
+          const int* e = o->ffoff[i];
+          if (WALL(o, x + e[0], y + e[1], z + e[2]) != 0) continue;
+          if (o->ffnalt[i] > 0) {
+            int any = 0;
+            for (int a = 0; a < o->ffnalt[i] && !any; a++) {
+              int all = 1;
+              for (int c = 0; c < o->ffaltlen[i][a]; c++) {
+                const int* q = o->ffalt[i][a][c];
+                if (WALL(o, x + q[0], y + q[1], z + q[2]) != 0) all = 0;
+              }
+              any = all;
+            }
+            if (!any) continue;
+          }
+          double fw = o->ffw[o->ffk[i]];
+          for (int d = 0; d < o->D; d++) {
+            int c = e[d];
+            if (c == 0) continue;
+            int ac = abs(c);
+            /* 2.*ffw*(drho) / 3.*ffw*(drho) for |e_d| > 1 */
+            double coef = ac == 1 ? fw : (double)ac * fw;
+            for (int m = 0; m < o->C; m++) {
+              double dr = RHO(o, x + e[0], y + e[1], z + e[2], m) - RHO(o, x, y, z, m);
+              if (c > 0)
+                grad[m][d] = grad[m][d] + coef * dr;
+              else
+                grad[m][d] = grad[m][d] - coef * dr;
+            }
+            ws[d] = ws[d] + (ac == 1 ? fw : fw * (double)(ac * ac));
+          }
+        }
+        for (int m = 0; m < o->C; m++)
+          for (int d = 0; d < o->D; d++) {
+            if (ws[d] > eps12) {
+              double s = 0.0;
+              for (int mp = 0; mp < o->C; mp++) s = s + o->d.g[m][mp] * (grad[mp][d] / ws[d]);
+              FRC(o, x, y, z, d, m) = FRC(o, x, y, z, d, m) - o->c0 * RHO(o, x, y, z, m) * s;
+            }
+          }
+      }
+}
+
+/* FlowCalcForces (lbm_flow.F90:760-808); the density halo ends here */
+static void calc_forces(orc_t* o) {
+  memset(o->forces, 0, sizeof(double) * (size_t)(o->gN * o->D * o->C));
+  if (o->fs) add_fluid_solid(o);
+  if (o->body) add_body(o);
+  if (o->ff) {
+    halo(o, o->rho, o->C, 1);
+    add_fluid_fluid(o);
+  }
+}
+
+/* FlowUpdateUED3/D2 (lbm_flow.F90:494-573) */
+static void update_ue(orc_t* o) {
+  double mmot[MAXC];
+  for (int m = 0; m < o->C; m++) mmot[m] = o->d.mm[m] * o->s_c[m];
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        if (WALL(o, x, y, z) != 0) continue;
+        double up[3];
+        for (int d = 0; d < o->D; d++)
+          for (int m = 0; m < o->C; m++)
+            FLUX(o, x, y, z, d, m) = FLUX(o, x, y, z, d, m) + .5 * FRC(o, x, y, z, d, m);
+        for (int d = 0; d < o->D; d++) {
+          double num = 0.0, den = 0.0;
+          for (int m = 0; m < o->C; m++) num = num + FLUX(o, x, y, z, d, m) * mmot[m];
+          for (int m = 0; m < o->C; m++) den = den + RHO(o, x, y, z, m) * mmot[m];
+          up[d] = num / den;
+        }
+        for (int m = 0; m < o->C; m++)
+          for (int d = 0; d < o->D; d++) FLUX(o, x, y, z, d, m) = up[d];
+      }
+}
+
+/* ------------------------------------------------------------ collision */
+/* DiscretizationEquilf_D3Q19 (lbm_discretization_d3q19.F90:265-305) and
+ * _D2Q9 (lbm_discretization_d2q9.F90:168-205), all components */
+static void update_feq(orc_t* o) {
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        if (WALL(o, x, y, z) != 0) continue;
+        for (int m = 0; m < o->C; m++) {
+          double usqr = 0.0;
+          for (int d = 0; d < o->D; d++) usqr = usqr + FLUX(o, x, y, z, d, m) * FLUX(o, x, y, z, d, m);
+          double r = RHO(o, x, y, z, m), dk = o->d_k[m], cs2 = o->c_s2[m];
+          if (o->D == 3)
+            FEQ(o, x, y, z, 0, m) = r * (dk - usqr / 2.);
+          else
+            FEQ(o, x, y, z, 0, m) = r * ((1. + dk * 5.) / 6. - 2. * usqr / 3.);
+          for (int n = 1; n <= o->b; n++) {
+            double udote = 0.0;
+            for (int d = 0; d < o->D; d++) udote = udote + (double)o->ci[n][d] * FLUX(o, x, y, z, d, m);
+            FEQ(o, x, y, z, n, m) = o->w[n] * r *
+                                    (1.5 * (1. - dk) + udote / cs2 + udote * udote / (2. * cs2 * cs2) -
+                                     usqr / (2. * cs2));
+          }
+        }
+      }
+}
+
+/* FlowFiBarEqPrefactor (lbm_flow.F90:836-851) */
+static inline void prefactor(const orc_t* o, int x, int y, int z, double pref[MAXC][19]) {
+  for (int n = 0; n <= o->b; n++)
+    for (int m = 0; m < o->C; m++) {
+      double s = 0.0;
+      for (int d = 0; d < o->D; d++)
+        s = s + FRC(o, x, y, z, d, m) * ((double)o->ci[n][d] - FLUX(o, x, y, z, d, m));
+      pref[m][n] = s / (RHO(o, x, y, z, m) * o->c_s2[m]);
+    }
+}
+
+/* FlowFeqBarD3/D2 (lbm_flow.F90:868-921) */
+static void fibar_init(orc_t* o) {
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        if (WALL(o, x, y, z) != 0) continue;
+        double pref[MAXC][19];
+        prefactor(o, x, y, z, pref);
+        for (int n = 0; n <= o->b; n++)
+          for (int m = 0; m < o->C; m++) FI(o, x, y, z, n, m) = (1. - 0.5 * pref[m][n]) * FEQ(o, x, y, z, n, m);
+      }
+}
+
+/* FlowCollisionD3/D2 (lbm_flow.F90:960-1029) with RelaxationCollideSRT/MRT
+ * (lbm_relaxation.F90:171-200) */
+static void collide(orc_t* o) {
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        if (WALL(o, x, y, z) != 0) continue;
+        double pref[MAXC][19], bar[MAXC][19];
+        prefactor(o, x, y, z, pref);
+        for (int m = 0; m < o->C; m++)
+          for (int n = 0; n <= o->b; n++) bar[m][n] = (1. - .5 * pref[m][n]) * FEQ(o, x, y, z, n, m);
+        for (int m = 0; m < o->C; m++) {
+          if (o->d.relaxation_mode == TXG_SRT) {
+            for (int n = 0; n <= o->b; n++)
+              FI(o, x, y, z, n, m) = FI(o, x, y, z, n, m) - (FI(o, x, y, z, n, m) - bar[m][n]) / o->tau[m];
+          } else {
+            double dfi[19];
+            for (int k = 0; k <= o->b; k++) dfi[k] = FI(o, x, y, z, k, m) - bar[m][k];
+            for (int n = 0; n <= o->b; n++) {
+              double mdiff = 0.0;
+              for (int k = 0; k <= o->b; k++) mdiff = mdiff + o->mt[n][k] * dfi[k];
+              for (int k = 0; k <= o->b; k++)
+                FI(o, x, y, z, k, m) =
+                    FI(o, x, y, z, k, m) - o->tau_mrt[m][n] * mdiff / o->mmt[n] * o->mt[n][k];
+            }
+          }
+        }
+        for (int n = 0; n <= o->b; n++)
+          for (int m = 0; m < o->C; m++)
+            FI(o, x, y, z, n, m) = FI(o, x, y, z, n, m) + pref[m][n] * FEQ(o, x, y, z, n, m);
+      }
+}
+
+/* ------------------------------------------------------------ streaming */
+/* BCPreStream_D2/D3 (lbm_bc.F90:613-779): Dirichlet faces copy the
+ * boundary node's outgoing-from-outside populations into the ghost source
+ * cells so that the stream delivers f*(x, t). */
+static void bc_prestream(orc_t* o) {
+  int lo[3] = {1, 1, 1}, hi[3] = {o->NX, o->NY, o->NZ};
+  for (int f = 0; f < 2 * o->D; f++) {
+    if (!o->dirichlet[f]) continue;
+    int dim = f / 2, plus = f % 2;
+    for (int n = 1; n <= o->b; n++) {
+      int c = o->ci[n][dim];
+      if (!((plus == 0 && c > 0) || (plus == 1 && c < 0))) continue;
+      int face = plus ? hi[dim] : lo[dim];
+      int a0[3], a1[3];
+      for (int d = 0; d < 3; d++) {
+        a0[d] = (d < o->D) ? lo[d] : 1;
+        a1[d] = (d < o->D) ? hi[d] : 1;
+      }
+      a0[dim] = a1[dim] = face;
+      for (int z = a0[2]; z <= a1[2]; z++)
+        for (int y = a0[1]; y <= a1[1]; y++)
+          for (int x = a0[0]; x <= a1[0]; x++)
+            for (int m = 0; m < o->C; m++)
+              FI(o, x - o->ci[n][0], y - o->ci[n][1], z - o->ci[n][2], n, m) = FI(o, x, y, z, n, m);
+    }
+  }
+}
+
+/* DistributionStreamD3/D2 (lbm_distribution_function.F90:560-667) */
+static void stream(orc_t* o) {
+  memset(o->tmp, 0, sizeof(double) * (size_t)(o->gN * o->Q * o->C));
+  int lo[3] = {1, 1, 1}, hi[3] = {o->NX, o->NY, o->NZ};
+  for (int n = 0; n <= o->b; n++) {
+    int d0[3], d1[3];
+    for (int d = 0; d < 3; d++) {
+      int c = (d < o->D) ? o->ci[n][d] : 0;
+      d0[d] = lo[d] + (c < 0 ? c : 0);
+      d1[d] = hi[d] + (c > 0 ? c : 0);
+    }
+#pragma omp parallel for schedule(static)
+    for (int z = d0[2]; z <= d1[2]; z++)
+      for (int y = d0[1]; y <= d1[1]; y++)
+        for (int x = d0[0]; x <= d1[0]; x++)
+          for (int m = 0; m < o->C; m++)
+            o->tmp[(FIX(o, x, y, z) * o->Q + n) * o->C + m] =
+                FI(o, x - o->ci[n][0], y - o->ci[n][1], z - o->ci[n][2], n, m);
+  }
+  double* t = o->fi;
+  o->fi = o->tmp;
+  o->tmp = t;
+}
+
+/* DistributionBouncebackD3/D2 (lbm_distribution_function.F90:669-784):
+ * scan the f-ghosted box in storage order */
+static void bounceback(orc_t* o) {
+  int lo[3] = {1, 1, 1}, hi[3] = {o->NX, o->NY, o->NZ};
+  for (int z = o->gzs; z < o->gzs + o->gzl; z++)
+    for (int y = o->gys; y < o->gys + o->gyl; y++)
+      for (int x = o->gxs; x < o->gxs + o->gxl; x++) {
+        double wv = WALL(o, x, y, z);
+        int mode = -1;
+        if (wv == WALL_NORMAL_X)
+          mode = 0;
+        else if (wv == WALL_NORMAL_Y)
+          mode = 1;
+        else if (o->D == 3 && wv == WALL_NORMAL_Z)
+          mode = 2;
+        else if (wv > 0)
+          mode = 3;
+        if (mode < 0) continue;
+        int c[3] = {x, y, z};
+        for (int n = 0; n <= o->b; n++) {
+          int t[3] = {x, y, z}, nn, ok = 1;
+          if (mode < 3) {
+            t[mode] = c[mode] - o->ci[n][mode];
+            ok = t[mode] >= lo[mode] && t[mode] <= hi[mode];
+            nn = o->refl[mode][n];
+          } else {
+            for (int d = 0; d < o->D; d++) {
+              t[d] = c[d] - o->ci[n][d];
+              if (t[d] < lo[d] || t[d] > hi[d]) ok = 0;
+            }
+            nn = o->opp[n];
+          }
+          if (ok)
+            for (int m = 0; m < o->C; m++) FI(o, t[0], t[1], t[2], nn, m) = FI(o, x, y, z, n, m);
+        }
+        for (int n = 0; n <= o->b; n++)
+          for (int m = 0; m < o->C; m++) FI(o, x, y, z, n, m) = 0.0;
+      }
+}
+
+/* ------------------------------------------------------------ Dirichlet BC */
+/* face node loops: the boundary layer of face f in the reference loop
+ * order (lbm_bc.F90:1116-1198) */
+static void face_range(const orc_t* o, int f, int a0[3], int a1[3]) {
+  int hi[3] = {o->NX, o->NY, o->NZ};
+  for (int d = 0; d < 3; d++) {
+    a0[d] = 1;
+    a1[d] = (d < o->D) ? hi[d] : 1;
+  }
+  int dim = f / 2;
+  a0[dim] = a1[dim] = (f % 2) ? hi[dim] : 1;
+}
+
+/* BCApplyDirichletToRho_D2/D3 (lbm_bc.F90:252-434) */
+static void bc_dirichlet_to_rho(orc_t* o) {
+  for (int f = 0; f < 2 * o->D; f++) {
+    if (!o->dirichlet[f]) continue;
+    int a0[3], a1[3];
+    face_range(o, f, a0, a1);
+    for (int z = a0[2]; z <= a1[2]; z++)
+      for (int y = a0[1]; y <= a1[1]; y++)
+        for (int x = a0[0]; x <= a1[0]; x++)
+          if (WALL(o, x, y, z) == 0)
+            for (int m = 0; m < o->C; m++) RHO(o, x, y, z, m) = o->d.bc_density[f][m];
+  }
+}
+
+/* DiscSetLocalDirections: inward local normal direction and the cardinal
+ * axes (normal, cross, resultant) per face
+ * (lbm_discretization_d3q19.F90:566-714, _d2q9.F90:372-434) */
+static void local_dirs(const orc_t* o, int f, int* ln, int card[3]) {
+  if (o->D == 3) {
+    static const int lnd[6] = {1, 3, 2, 4, 5, 6}; /* EAST, WEST, NORTH, SOUTH, UP, DOWN */
+    static const int cd[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+    *ln = lnd[f];
+    for (int i = 0; i < 3; i++) card[i] = cd[f / 2][i];
+  } else {
+    static const int lnd[4] = {1, 3, 2, 4};
+    static const int cd[2][2] = {{0, 1}, {1, 0}};
+    *ln = lnd[f];
+    card[0] = cd[f / 2][0];
+    card[1] = cd[f / 2][1];
+    card[2] = -1;
+  }
+}
+
+/* BCApplyDirichletNode (lbm_bc.F90:1273-1333) */
+static void dirichlet_node(orc_t* o, int x, int y, int z, int f) {
+  int ln, card[3];
+  local_dirs(o, f, &ln, card);
+  int nrm = card[0];
+  int cl = o->ci[ln][nrm];
+  for (int m = 0; m < o->C; m++) {
+    double wsum[3] = {0., 0., 0.}, Q[3] = {0., 0., 0.}, mom[3] = {0., 0., 0.};
+    for (int n = 1; n <= o->b; n++)
+      if (cl * o->ci[n][nrm] == 1) wsum[nrm] = wsum[nrm] + o->w[n];
+    double s = 0.0;
+    for (int n = 0; n <= o->b; n++) s = s + FI(o, x, y, z, n, m);
+    Q[nrm] = (double)cl * (o->d.bc_density[f][m] - s) / wsum[nrm];
+    for (int p = 1; p < o->D; p++) {
+      int cp = card[p];
+      for (int n = 1; n <= o->b; n++) {
+        mom[cp] = mom[cp] + FI(o, x, y, z, n, m) * (double)o->ci[n][cp];
+        if (cl * o->ci[n][nrm] == 1 && o->ci[n][cp] != 0) wsum[cp] = wsum[cp] + o->w[n];
+      }
+      Q[cp] = -mom[cp] / wsum[cp];
+    }
+    for (int n = 1; n <= o->b; n++)
+      if (cl * o->ci[n][nrm] == 1) {
+        double cq = 0.0;
+        for (int d = 0; d < o->D; d++) cq = cq + (double)o->ci[n][d] * Q[d];
+        FI(o, x, y, z, n, m) = FI(o, x, y, z, n, m) + o->w[n] * cq;
+      }
+  }
+}
+
+/* BCApplyDirichletD2/D3 (lbm_bc.F90:1090-1271) */
+static void bc_apply(orc_t* o) {
+  for (int f = 0; f < 2 * o->D; f++) {
+    if (!o->dirichlet[f]) continue;
+    int a0[3], a1[3];
+    face_range(o, f, a0, a1);
+    for (int z = a0[2]; z <= a1[2]; z++)
+      for (int y = a0[1]; y <= a1[1]; y++)
+        for (int x = a0[0]; x <= a1[0]; x++)
+          if (WALL(o, x, y, z) == 0) dirichlet_node(o, x, y, z, f);
+  }
+}
+
+/* BCUpdateRho_D2/D3 (lbm_bc.F90:436-611) */
+static void bc_update_rho(orc_t* o) {
+  for (int f = 0; f < 2 * o->D; f++) {
+    if (!o->dirichlet[f]) continue;
+    int a0[3], a1[3];
+    face_range(o, f, a0, a1);
+    for (int z = a0[2]; z <= a1[2]; z++)
+      for (int y = a0[1]; y <= a1[1]; y++)
+        for (int x = a0[0]; x <= a1[0]; x++)
+          if (WALL(o, x, y, z) == 0)
+            for (int m = 0; m < o->C; m++) {
+              double s = 0.0;
+              for (int n = 0; n <= o->b; n++) s = s + FI(o, x, y, z, n, m);
+              RHO(o, x, y, z, m) = s;
+            }
+  }
+}
+
+/* ------------------------------------------------------------ driver */
+/* LBMInit2 flow part (lbm.F90:183-253): FlowFiInit (lbm_flow.F90:923-934)
+ * unless fi came from a file, then FlowUpdateMoments (:466-478). */
+int orc_init(orc_t* o) {
+  if (!o->fi_loaded) {
+    calc_forces(o); /* density halo inside */
+    update_feq(o);
+    fibar_init(o);
+  }
+  calc_density(o);
+  calc_flux_into(o, o->flux);
+  calc_forces(o);
+  update_ue(o);
+  o->initialized = 1;
+  return 0;
+}
+
+/* LBMRun2 flow body (lbm.F90:279-406) */
+int orc_step(orc_t* o, int nsteps) {
+  if (!o->initialized) return TXG_ERR_STATE;
+  for (int s = 0; s < nsteps; s++) {
+    update_feq(o);      /* FlowCollision */
+    collide(o);
+    halo(o, o->fi, o->Q * o->C, 0); /* DistributionCommunicateFi */
+    bc_prestream(o);    /* FlowStream */
+    stream(o);
+    bounceback(o);      /* FlowBounceback */
+    calc_density(o);    /* FlowApplyBCs -> FlowCalcRhoForces */
+    bc_dirichlet_to_rho(o);
+    calc_forces(o);
+    bc_apply(o);
+    bc_update_rho(o);
+    calc_flux_into(o, o->flux); /* FlowUpdateFlux */
+    update_ue(o);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------ outputs */
+int orc_get_fi(const orc_t* o, double* fi) {
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        long node = (x - 1) + (long)o->NX * ((y - 1) + (long)o->NY * (z - 1));
+        for (int n = 0; n < o->Q; n++)
+          for (int m = 0; m < o->C; m++) fi[(node * o->Q + n) * o->C + m] = FI(o, x, y, z, n, m);
+      }
+  return 0;
+}
+
+int orc_get_moments(const orc_t* o, double* rho, double* flux, double* forces) {
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        long node = (x - 1) + (long)o->NX * ((y - 1) + (long)o->NY * (z - 1));
+        for (int m = 0; m < o->C; m++) {
+          if (rho) rho[node * o->C + m] = RHO(o, x, y, z, m);
+          for (int d = 0; d < o->D; d++) {
+            if (flux) flux[(node * o->D + d) * o->C + m] = FLUX(o, x, y, z, d, m);
+            if (forces) forces[(node * o->D + d) * o->C + m] = FRC(o, x, y, z, d, m);
+          }
+        }
+      }
+  return 0;
+}
+
+/* FlowUpdateDiagnosticsD3/D2 (lbm_flow.F90:603-758), psi = rho */
+int orc_diagnostics(orc_t* o, double* rhot, double* prs, double* velt) {
+  long nf = o->gN * o->D * o->C;
+  double* u = (double*)malloc(sizeof(double) * (size_t)nf);
+  if (!u) return TXG_ERR_NOMEM;
+  memset(u, 0, sizeof(double) * (size_t)nf);
+  calc_flux_into(o, u);
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        long node = (x - 1) + (long)o->NX * ((y - 1) + (long)o->NY * (z - 1));
+        long base = FIX(o, x, y, z) * o->D * o->C;
+        if (WALL(o, x, y, z) == 0) {
+          double rt = 0.0;
+          for (int m = 0; m < o->C; m++) rt = rt + RHO(o, x, y, z, m) * o->d.mm[m];
+          double p = rt / 3.;
+          if (o->C > 1) {
+            for (int m = 0; m < o->C; m++) {
+              double s = 0.0;
+              for (int mp = 0; mp < o->C; mp++) s = s + o->d.g[m][mp] * RHO(o, x, y, z, mp);
+              p = p + o->c0 / 2. * RHO(o, x, y, z, m) * s;
+            }
+          }
+          for (int m = 0; m < o->C; m++)
+            for (int d = 0; d < o->D; d++)
+              u[base + d * o->C + m] = u[base + d * o->C + m] + .5 * FRC(o, x, y, z, d, m);
+          if (rhot) rhot[node] = rt;
+          if (prs) prs[node] = p;
+          for (int d = 0; d < o->D; d++) {
+            double s = 0.0;
+            for (int m = 0; m < o->C; m++) s = s + u[base + d * o->C + m] * o->d.mm[m];
+            if (velt) velt[node * o->D + d] = s / rt;
+          }
+        } else {
+          if (prs) prs[node] = 0.0; /* flow%null_pressure */
+          if (rhot) rhot[node] = 0.0;
+          if (velt)
+            for (int d = 0; d < o->D; d++) velt[node * o->D + d] = 0.0;
+        }
+      }
+  free(u);
+  return 0;
+}
+
+/* DistributionCalcDeltaNorm on rho (lbm_distribution_function.F90:809-833);
+ * rho_old starts at zero (VecSet, :173-174) */
+int orc_delta_norm(orc_t* o, double* norm) {
+  double mx = 0.0;
+  long i = 0;
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++)
+        for (int m = 0; m < o->C; m++, i++) {
+          double r = RHO(o, x, y, z, m);
+          double v = o->rho_old[i] + -1.0 * r;
+          double q = (r != 0.0) ? v / r : 0.0;
+          if (fabs(q) > mx) mx = fabs(q);
+          o->rho_old[i] = r;
+        }
+  *norm = mx;
+  return 0;
+}
+
+/* ------------------------------------------------------------ phase hooks
+ * Run one reference phase by name, for phase-level tests:
+ * 0 feq+collide, 1 fi halo, 2 prestream+stream, 3 bounceback,
+ * 4 density, 5 dirichlet-to-rho, 6 forces, 7 bc apply, 8 bc update rho,
+ * 9 flux, 10 common velocity, 11 feq only, 12 fi = feq (no forcing). */
+int orc_phase(orc_t* o, int phase) {
+  switch (phase) {
+    case 0: update_feq(o); collide(o); break;
+    case 1: halo(o, o->fi, o->Q * o->C, 0); break;
+    case 2: bc_prestream(o); stream(o); break;
+    case 3: bounceback(o); break;
+    case 4: calc_density(o); break;
+    case 5: bc_dirichlet_to_rho(o); break;
+    case 6: calc_forces(o); break;
+    case 7: bc_apply(o); break;
+    case 8: bc_update_rho(o); break;
+    case 9: calc_flux_into(o, o->flux); break;
+    case 10: update_ue(o); break;
+    case 11: update_feq(o); break;
+    case 12:
+      update_feq(o);
+      memcpy(o->fi, o->fi_eq, sizeof(double) * (size_t)(o->gN * o->Q * o->C));
+      break;
+    default: return TXG_ERR_ARG;
+  }
+  o->initialized = 1;
+  return 0;
+}

@@ -1,0 +1,29 @@
+// txg_dispatch.h -- type-erased launchers for one (lattice, components,
+// isotropy, precision) configuration of the step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace txg {
+
+struct ExportPtrs {
+  double* f;
+  double* rho;
+  double* up;
+  double* F;
+  double* j;
+};
+
+struct KernelSet {
+  // p: Params<R>*, b: Bufs<S,R>*
+  hipError_t (*density)(const void* p, const void* b, int pulled, int apply_bc, int N, hipStream_t s);
+  hipError_t (*collide)(const void* p, const void* b, int pulled, int apply_bc, const ExportPtrs* ex, int N,
+                        hipStream_t s);
+  hipError_t (*fiinit)(const void* p, const void* b, const void* u_ic, int N, hipStream_t s);
+};
+
+// returns false if the configuration is not instantiated
+bool kernels_fp64(int dims, int ncomp, int iso, KernelSet* ks);
+bool kernels_fp32(int dims, int ncomp, int iso, bool half_storage, KernelSet* ks);
+
+}  // namespace txg

@@ -1,0 +1,749 @@
+// txg_kernels.h -- HIP kernels of the Shan-Chen flow step for gfx950.
+//
+// One timestep of the reference (LBMRun2, lbm.F90:279-406) is
+//   collide -> halo -> stream -> bounce-back -> BCs -> rho -> forces -> u'.
+// On the device the populations are kept post-collision (f*), and a step is
+// two passes over the lattice:
+//   k_density : pull f(x) = f*(x - c_n) (bounce-back where x - c_n is solid),
+//               rho = sum f, Dirichlet rho on boundary faces   -> rho[]
+//   k_collide : pull f again, Dirichlet node correction, moments, fluid-solid,
+//               body and psi-gradient forces (stencil on rho[]), common
+//               velocity, equilibrium, BGK/MRT collision       -> f*(t+1)
+// so the stored state is read twice and written once per step.
+//
+// Layout: populations are SoA, f[(m*Q + n)*Np + node], node = x + NX*(y + NY*z);
+// rho is SoA rho[m*Np + node] with NaN at solid nodes (the wall test of the
+// gradient stencil is "neighbour rho is a number"); per-node link masks carry
+// the precomputed bounce-back links.
+//
+// Arithmetic: every expression is written in the reference's evaluation
+// order (lbm_flow.F90, lbm_forcing.F90, lbm_relaxation.F90,
+// lbm_discretization_*.F90, lbm_bc.F90); the FP64 instantiation is compiled
+// with -ffp-contract=off and reproduces the reference's FP64 results bit for
+// bit.  The FP32 instantiations use the same code in float (contraction on).
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+#include <utility>
+
+#include "txg_lattice.h"
+
+namespace txg {
+
+constexpr int MAXC = 4;
+constexpr int MAXMIN = 16;
+
+// per-node link mask bits
+constexpr uint32_t kSelfSolid = 1u << 31;   // node is not pore space
+constexpr uint32_t kFSNear = 1u << 19;      // a neighbour x + c_n is a mineral wall
+constexpr uint32_t kFFClean = 1u << 20;     // no wall inside the gradient stencil reach
+constexpr int kFaceShift = 24;               // bits 24..29: node on a Dirichlet face XM..ZP
+// bits 1..Q-1: the pull source x - c_n is a wall -> bounce-back
+
+// flags
+constexpr int kFlagFF = 1, kFlagFS = 2, kFlagBody = 4, kFlagMRT = 8;
+
+template <class R>
+struct Params {
+  int nx, ny, nz, nxy;
+  int N;     // nodes
+  int Np;    // plane stride (>= N)
+  int periodic[3];
+  int flags;
+  int nminerals;
+  int dirichlet;  // bit f set: face f has a density BC
+  R tau[MAXC];
+  R mmot[MAXC];        // mm * s_c (lbm_flow.F90:518-520)
+  R mm[MAXC];
+  R dk[MAXC];          // d_k = 1 - 2/(3 mm)
+  R cs2[MAXC];         // c_s^2 = 1/3
+  R a15[MAXC];         // 1.5*(1 - d_k)
+  R two_cs2[MAXC];     // 2*c_s2
+  R two_cs2_sq[MAXC];  // 2*c_s2*c_s2
+  R feq0a[MAXC];       // D2Q9: (1 + d_k*5)/6
+  R rate[MAXC][7];     // s_c s_e s_e2 s_q s_nu s_pi s_m
+  R gvt[3];
+  R g[MAXC][MAXC];
+  R gw[MAXMIN][MAXC];
+  R fw[11], fw2[11], fw3[11], fwt4[11], fwt9[11];  // ffw, 2ffw, 3ffw, ffw*4, ffw*9
+  R c0;
+  R ffeps;             // 1.e-12 as a default-real literal
+  R kax, kdg;          // fluid-solid 1./6. (1./3.) and 1./12. as default reals
+  R bcrho[6][MAXC];
+  R shift0[MAXC];      // FP16 storage: stored = f - w_n * shift0[m]
+};
+
+template <class S, class R>
+struct Bufs {
+  const S* __restrict__ fin;
+  S* __restrict__ fout;
+  R* __restrict__ rho;
+  const uint32_t* __restrict__ mask;
+  const uint8_t* __restrict__ tag;
+};
+
+// ------------------------------------------------------------ storage
+template <class S, class R>
+__device__ __forceinline__ R ld(const S* p, int i, R shift) {
+  if constexpr (std::is_same<S, __half>::value)
+    return (R)__half2float(p[i]) + shift;
+  else
+    return (R)p[i];
+}
+template <class S, class R>
+__device__ __forceinline__ void st(S* p, int i, R v, R shift) {
+  if constexpr (std::is_same<S, __half>::value)
+    p[i] = __float2half_rn((float)(v - shift));
+  else
+    p[i] = (S)v;
+}
+
+// compile-time loop
+template <int B, int E, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    sfor<B + 1, E>(f);
+  }
+}
+
+// ------------------------------------------------------------ geometry
+// Wrapped neighbour rows of a node: xw[3+k] = linear offset contribution of
+// x+k (or -1 when x+k leaves a non-periodic domain), likewise y and z.
+struct Nbr {
+  int xw[7], yw[7], zw[7];
+};
+
+template <int D, class R>
+__device__ __forceinline__ void make_nbr(const Params<R>& p, int x, int y, int z, Nbr& nb) {
+#pragma unroll
+  for (int k = -3; k <= 3; k++) {
+    int xx = x + k, yy = y + k, zz = z + k;
+    if (xx < 0) xx = p.periodic[0] ? xx + p.nx : -1;
+    else if (xx >= p.nx) xx = p.periodic[0] ? xx - p.nx : -1;
+    if (yy < 0) yy = p.periodic[1] ? yy + p.ny : -1;
+    else if (yy >= p.ny) yy = p.periodic[1] ? yy - p.ny : -1;
+    nb.xw[k + 3] = xx;
+    nb.yw[k + 3] = yy < 0 ? -1 : yy * p.nx;
+    if constexpr (D == 3) {
+      if (zz < 0) zz = p.periodic[2] ? zz + p.nz : -1;
+      else if (zz >= p.nz) zz = p.periodic[2] ? zz - p.nz : -1;
+      nb.zw[k + 3] = zz < 0 ? -1 : zz * p.nxy;
+    } else {
+      nb.zw[k + 3] = (k == 0) ? 0 : -1;
+    }
+  }
+}
+
+// neighbour index at offset (ex,ey,ez), or -1 outside a non-periodic domain
+template <int EX, int EY, int EZ>
+__device__ __forceinline__ int nidx(const Nbr& nb) {
+  int a = nb.xw[EX + 3], b = nb.yw[EY + 3], c = nb.zw[EZ + 3];
+  return (a < 0 || b < 0 || c < 0) ? -1 : a + b + c;
+}
+
+// ------------------------------------------------------------ pull-stream
+// f_n(x) = f*_n(x - c_n), or f*_opp(n)(x) when x - c_n is a wall
+// (DistributionStream + DistributionBounceback, lbm_distribution_function.F90:560-784)
+template <int D, int C, class S, class R>
+__device__ __forceinline__ void pull(const Params<R>& p, const S* __restrict__ fin, int node,
+                                     uint32_t mk, const Nbr& nb, R (&f)[C][Lat<D>::Q]) {
+  using L = Lat<D>;
+  sfor<0, L::Q>([&](auto NI) {
+    constexpr int n = decltype(NI)::value;
+    int src;
+    int dir;
+    if (n > 0 && (mk & (1u << n))) {
+      src = node;
+      dir = L::opp[n];
+    } else {
+      src = nidx<-L::cx[n], -L::cy[n], -L::cz[n]>(nb);
+      dir = n;
+    }
+    // w_opp(n) = w_n, so the storage shift does not depend on the branch
+#pragma unroll
+    for (int m = 0; m < C; m++)
+      f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + dir) * p.Np, src, (R)L::w[n] * p.shift0[m]);
+  });
+}
+
+// read the populations of node without streaming (initial state)
+template <int D, int C, class S, class R>
+__device__ __forceinline__ void load_local(const Params<R>& p, const S* __restrict__ fin, int node,
+                                           R (&f)[C][Lat<D>::Q]) {
+  using L = Lat<D>;
+#pragma unroll
+  for (int n = 0; n < L::Q; n++)
+#pragma unroll
+    for (int m = 0; m < C; m++) f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + n) * p.Np, node, (R)L::w[n] * p.shift0[m]);
+}
+
+// ------------------------------------------------------------ Dirichlet BC
+// inward local normal direction of each face (DiscSetLocalDirections,
+// lbm_discretization_d3q19.F90:566-714, _d2q9.F90:372-434) and the axes
+// (normal, cross, resultant)
+template <int D>
+struct FaceDirs;
+template <>
+struct FaceDirs<3> {
+  static constexpr int ln[6] = {1, 3, 2, 4, 5, 6};
+  static constexpr int card[6][3] = {{0, 1, 2}, {0, 1, 2}, {1, 2, 0}, {1, 2, 0}, {2, 0, 1}, {2, 0, 1}};
+};
+template <>
+struct FaceDirs<2> {
+  static constexpr int ln[4] = {1, 3, 2, 4};
+  static constexpr int card[4][3] = {{0, 1, 2}, {0, 1, 2}, {1, 0, 2}, {1, 0, 2}};
+};
+
+template <int D>
+__host__ __device__ constexpr int cdir(int n, int d) {
+  return d == 0 ? Lat<D>::cx[n] : (d == 1 ? Lat<D>::cy[n] : Lat<D>::cz[n]);
+}
+
+// BCApplyDirichletNode (lbm_bc.F90:1273-1333) for face F
+template <int D, int C, int F, class R>
+__device__ __forceinline__ void dirichlet_node(const Params<R>& p, R (&f)[C][Lat<D>::Q]) {
+  using L = Lat<D>;
+  constexpr int ln = FaceDirs<D>::ln[F];
+  constexpr int nrm = FaceDirs<D>::card[F][0];
+  constexpr int cl = cdir<D>(ln, nrm);
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R wsum[3] = {(R)0, (R)0, (R)0}, Q[3] = {(R)0, (R)0, (R)0}, mom[3] = {(R)0, (R)0, (R)0};
+    sfor<1, L::Q>([&](auto NI) {
+      constexpr int n = decltype(NI)::value;
+      if constexpr (cl * cdir<D>(n, nrm) == 1) wsum[nrm] = wsum[nrm] + (R)L::w[n];
+    });
+    R s = (R)0;
+#pragma unroll
+    for (int n = 0; n < L::Q; n++) s = s + f[m][n];
+    Q[nrm] = (R)cl * (p.bcrho[F][m] - s) / wsum[nrm];
+    sfor<1, D>([&](auto PI) {
+      constexpr int cp = FaceDirs<D>::card[F][decltype(PI)::value];
+      sfor<1, L::Q>([&](auto NI) {
+        constexpr int n = decltype(NI)::value;
+        constexpr int c = cdir<D>(n, cp);
+        if constexpr (c == 1) mom[cp] = mom[cp] + f[m][n];
+        if constexpr (c == -1) mom[cp] = mom[cp] - f[m][n];
+        if constexpr (cl * cdir<D>(n, nrm) == 1 && c != 0) wsum[cp] = wsum[cp] + (R)L::w[n];
+      });
+      Q[cp] = -mom[cp] / wsum[cp];
+    });
+    sfor<1, L::Q>([&](auto NI) {
+      constexpr int n = decltype(NI)::value;
+      if constexpr (cl * cdir<D>(n, nrm) == 1) {
+        R cq = (R)0;
+        sfor<0, D>([&](auto DI) {
+          constexpr int d = decltype(DI)::value;
+          constexpr int c = cdir<D>(n, d);
+          if constexpr (c == 1) cq = cq + Q[d];
+          if constexpr (c == -1) cq = cq - Q[d];
+        });
+        f[m][n] = f[m][n] + (R)L::w[n] * cq;
+      }
+    });
+  }
+}
+
+template <int D, int C, class R>
+__device__ __forceinline__ void apply_dirichlet(const Params<R>& p, uint32_t mk, R (&f)[C][Lat<D>::Q]) {
+  sfor<0, 2 * D>([&](auto FI) {
+    constexpr int F = decltype(FI)::value;
+    if (mk & (1u << (kFaceShift + F))) dirichlet_node<D, C, F, R>(p, f);
+  });
+}
+
+// ------------------------------------------------------------ forces
+// wall-test alternatives of a stencil offset (the rule the reference's
+// conditions follow, lbm_forcing.F90:314-939 and :1076-1285)
+struct Cond {
+  int nalt;
+  int len[4];
+  int c[4][2][3];
+};
+__host__ __device__ constexpr int isgn(int v) { return (v > 0) - (v < 0); }
+__host__ __device__ constexpr int iabs(int v) { return v < 0 ? -v : v; }
+__host__ __device__ constexpr Cond make_cond(int e0, int e1, int e2) {
+  Cond r{0, {0, 0, 0, 0}, {}};
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 2; b++)
+      for (int d = 0; d < 3; d++) r.c[a][b][d] = 0;
+  int e[3] = {e0, e1, e2};
+  int a[3] = {iabs(e0), iabs(e1), iabs(e2)};
+  int L = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+  int mx = a[0] > a[1] ? (a[0] > a[2] ? 0 : 2) : (a[1] > a[2] ? 1 : 2);
+  if (L <= 3) return r;
+  if (L == 4 || L == 8) {
+    r.nalt = 1;
+    r.len[0] = 1;
+    for (int d = 0; d < 3; d++) r.c[0][0][d] = e[d] / 2;
+    return r;
+  }
+  if (L == 5 || L == 6) {
+    int h[3] = {0, 0, 0};
+    for (int d = 0; d < 3; d++) h[d] = (d == mx) ? isgn(e[d]) : e[d];
+    int ones[2] = {0, 0};
+    int no = 0;
+    for (int d = 0; d < 3; d++)
+      if (d != mx && e[d] != 0) ones[no++] = d;
+    for (int s = 0; s < (1 << no); s++) {
+      r.len[r.nalt] = 1;
+      for (int d = 0; d < 3; d++) r.c[r.nalt][0][d] = h[d];
+      for (int q = 0; q < no; q++)
+        if (s & (1 << q)) r.c[r.nalt][0][ones[q]] = 0;
+      r.nalt++;
+    }
+    return r;
+  }
+  if (L == 9) {
+    r.nalt = 1;
+    r.len[0] = 2;
+    for (int d = 0; d < 3; d++) {
+      r.c[0][0][d] = isgn(e[d]);
+      r.c[0][1][d] = 2 * isgn(e[d]);
+    }
+    return r;
+  }
+  if (L == 10) {
+    r.nalt = 2;
+    for (int q = 0; q < 2; q++) {
+      r.len[q] = 2;
+      for (int d = 0; d < 3; d++) {
+        r.c[q][0][d] = (d == mx) ? isgn(e[d]) : (q ? e[d] : 0);
+        r.c[q][1][d] = (d == mx) ? 2 * isgn(e[d]) : (q ? e[d] : 0);
+      }
+    }
+  }
+  return r;
+}
+
+template <int D, int I>
+struct FFOff {
+  static constexpr int x = D == 3 ? kFF3[I][0] : kFF2[I][0];
+  static constexpr int y = D == 3 ? kFF3[I][1] : kFF2[I][1];
+  static constexpr int z = D == 3 ? kFF3[I][2] : 0;
+  static constexpr int k = x * x + y * y + z * z;
+  static constexpr Cond cond = make_cond(x, y, z);
+};
+
+template <class R>
+__device__ __forceinline__ bool is_num(R v) {
+  return v == v;
+}
+
+template <class R>
+__device__ __forceinline__ bool fluid_at(const R* __restrict__ rho0, int idx) {
+  return idx >= 0 && is_num(rho0[idx]);
+}
+
+// psi-gradient (fluid-fluid) force, LBMAddFluidFluidForcesD3/D2
+// (lbm_forcing.F90:51-1299), psi = rho
+template <int D, int C, int NFF, class R>
+__device__ __forceinline__ void ff_force(const Params<R>& p, const R* __restrict__ rho, int node,
+                                         const Nbr& nb, bool clean, const R (&rs)[C], R (&F)[C][3]) {
+  R grad[C][3];
+  R ws[3] = {(R)0, (R)0, (R)0};
+#pragma unroll
+  for (int m = 0; m < C; m++) grad[m][0] = grad[m][1] = grad[m][2] = (R)0;
+  sfor<0, NFF>([&](auto II) {
+    constexpr int I = decltype(II)::value;
+    using E = FFOff<D, I>;
+    const int t = nidx<E::x, E::y, E::z>(nb);
+    bool ok;
+    if (clean) {
+      ok = true;
+    } else {
+      ok = fluid_at(rho, t);
+      if constexpr (E::cond.nalt > 0) {
+        bool any = false;
+        sfor<0, E::cond.nalt>([&](auto AI) {
+          constexpr int a = decltype(AI)::value;
+          bool all = fluid_at(rho, nidx<E::cond.c[a][0][0], E::cond.c[a][0][1], E::cond.c[a][0][2]>(nb));
+          if constexpr (E::cond.len[a] > 1)
+            all = all && fluid_at(rho, nidx<E::cond.c[a][1][0], E::cond.c[a][1][1], E::cond.c[a][1][2]>(nb));
+          any = any || all;
+        });
+        ok = ok && any;
+      }
+    }
+    if (ok) {
+      R dr[C];
+#pragma unroll
+      for (int m = 0; m < C; m++) dr[m] = rho[(size_t)m * p.Np + t] - rs[m];
+      sfor<0, D>([&](auto DI) {
+        constexpr int d = decltype(DI)::value;
+        constexpr int c = d == 0 ? E::x : (d == 1 ? E::y : E::z);
+        if constexpr (c != 0) {
+          constexpr int ac = c < 0 ? -c : c;
+          const R coef = ac == 1 ? p.fw[E::k] : (ac == 2 ? p.fw2[E::k] : p.fw3[E::k]);
+#pragma unroll
+          for (int m = 0; m < C; m++) {
+            if constexpr (c > 0)
+              grad[m][d] = grad[m][d] + coef * dr[m];
+            else
+              grad[m][d] = grad[m][d] - coef * dr[m];
+          }
+          ws[d] = ws[d] + (ac == 1 ? p.fw[E::k] : (ac == 2 ? p.fwt4[E::k] : p.fwt9[E::k]));
+        }
+      });
+    }
+  });
+#pragma unroll
+  for (int m = 0; m < C; m++)
+#pragma unroll
+    for (int d = 0; d < D; d++)
+      if (ws[d] > p.ffeps) {
+        R s = (R)0;
+#pragma unroll
+        for (int mp = 0; mp < C; mp++) s = s + p.g[m][mp] * (grad[mp][d] / ws[d]);
+        F[m][d] = F[m][d] - p.c0 * rs[m] * s;
+      }
+}
+
+// fluid-solid (wettability) force, LBMAddFluidSolidForcesD3/D2
+// (lbm_forcing.F90:1326-1421): for each neighbour x + c_n that is a mineral
+template <int D, int C, class R>
+__device__ __forceinline__ void fs_force(const Params<R>& p, const uint8_t* __restrict__ tag, const Nbr& nb,
+                                         const R (&rs)[C], R (&F)[C][3]) {
+  using L = Lat<D>;
+  sfor<1, L::Q>([&](auto NI) {
+    constexpr int n = decltype(NI)::value;
+    const int t = nidx<L::cx[n], L::cy[n], L::cz[n]>(nb);
+    const int mineral = t >= 0 ? (int)tag[t] : 0;
+    if (mineral > 0) {
+      const R k = n <= L::NAX ? p.kax : p.kdg;
+      sfor<0, D>([&](auto DI) {
+        constexpr int d = decltype(DI)::value;
+        constexpr int c = cdir<D>(n, d);
+        if constexpr (c != 0) {
+#pragma unroll
+          for (int m = 0; m < C; m++) {
+            R v = k * rs[m] * p.gw[mineral - 1][m];
+            if constexpr (c > 0)
+              F[m][d] = F[m][d] - v;
+            else
+              F[m][d] = F[m][d] + v;
+          }
+        }
+      });
+    }
+  });
+}
+
+// ------------------------------------------------------------ kernels
+template <class R>
+__device__ __forceinline__ void node_xyz(const Params<R>& p, int node, int& x, int& y, int& z) {
+  z = node / p.nxy;
+  int r = node - z * p.nxy;
+  y = r / p.nx;
+  x = r - y * p.nx;
+}
+
+// rho for the gradient stencil: sum f after stream and bounce-back, then the
+// Dirichlet density on boundary nodes (DistributionCalcDensity +
+// BCApplyDirichletToRho, lbm_flow.F90:445-456)
+template <int D, int C, class S, class R>
+__global__ void __launch_bounds__(256) k_density(Params<R> p, Bufs<S, R> b, int pulled, int apply_bc) {
+  using L = Lat<D>;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= p.N) return;
+  const uint32_t mk = b.mask[node];
+  if (mk & kSelfSolid) return;
+  R f[C][L::Q];
+  if (pulled) {
+    int x, y, z;
+    node_xyz(p, node, x, y, z);
+    Nbr nb;
+    make_nbr<D>(p, x, y, z, nb);
+    pull<D, C, S, R>(p, b.fin, node, mk, nb, f);
+  } else {
+    load_local<D, C, S, R>(p, b.fin, node, f);
+  }
+  R rho[C];
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R s = (R)0;
+#pragma unroll
+    for (int n = 0; n < L::Q; n++) s = s + f[m][n];
+    rho[m] = s;
+  }
+  if (apply_bc && (mk & (0x3fu << kFaceShift))) {
+#pragma unroll
+    for (int fc = 0; fc < 2 * D; fc++)
+      if (mk & (1u << (kFaceShift + fc)))
+#pragma unroll
+        for (int m = 0; m < C; m++) rho[m] = p.bcrho[fc][m];
+  }
+#pragma unroll
+  for (int m = 0; m < C; m++) b.rho[(size_t)m * p.Np + node] = rho[m];
+}
+
+// Moments, forces, common velocity and collision of one node.
+// mode 0: step (pull from fin, Dirichlet BCs, collide into fout)
+// mode 1: initial collision (local read, no BCs)
+// mode 2: state export (pull + BCs unless local, no collision; writes f, rho,
+//         u', F, j in FP64 to the export arrays)
+template <int D, int C, class R>
+struct Export {
+  double* f;     // [(m*Q+n)*N + node]
+  double* rho;   // [m*N + node]
+  double* up;    // [d*N + node]
+  double* F;     // [(m*D+d)*N + node]
+  double* j;     // [(m*D+d)*N + node]
+};
+
+template <int D, int C, int NFF, class S, class R>
+__global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int pulled, int apply_bc,
+                                                 Export<D, C, R> ex) {
+  using L = Lat<D>;
+  constexpr int Q = L::Q;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= p.N) return;
+  const uint32_t mk = b.mask[node];
+  if (mk & kSelfSolid) {
+    if (ex.f) {  // solid nodes export zeros (rho = 0, DistributionCalcDensity)
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+#pragma unroll
+        for (int n = 0; n < Q; n++) ex.f[(size_t)(m * Q + n) * p.N + node] = 0.0;
+        ex.rho[(size_t)m * p.N + node] = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          ex.F[(size_t)(m * D + d) * p.N + node] = 0.0;
+          ex.j[(size_t)(m * D + d) * p.N + node] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < D; d++) ex.up[(size_t)d * p.N + node] = 0.0;
+    }
+    return;
+  }
+  int x, y, z;
+  node_xyz(p, node, x, y, z);
+  Nbr nb;
+  make_nbr<D>(p, x, y, z, nb);
+  R f[C][Q];
+  if (pulled)
+    pull<D, C, S, R>(p, b.fin, node, mk, nb, f);
+  else
+    load_local<D, C, S, R>(p, b.fin, node, f);
+  if (apply_bc && (mk & (0x3fu << kFaceShift))) apply_dirichlet<D, C, R>(p, mk, f);
+
+  // rho after BCUpdateRho, and j (DistributionCalcDensity/CalcFlux)
+  R rl[C], j[C][3];
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R s = (R)0;
+#pragma unroll
+    for (int n = 0; n < Q; n++) s = s + f[m][n];
+    rl[m] = s;
+    sfor<0, D>([&](auto DI) {
+      constexpr int d = decltype(DI)::value;
+      R a = (R)0;
+      sfor<0, Q>([&](auto NI) {
+        constexpr int n = decltype(NI)::value;
+        constexpr int c = cdir<D>(n, d);
+        if constexpr (c == 1) a = a + f[m][n];
+        if constexpr (c == -1) a = a - f[m][n];
+      });
+      j[m][d] = a;
+    });
+  }
+  // forces from the stencil rho (FlowCalcForces, lbm_flow.F90:760-808)
+  R rs[C];
+#pragma unroll
+  for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
+  R F[C][3];
+#pragma unroll
+  for (int m = 0; m < C; m++) F[m][0] = F[m][1] = F[m][2] = (R)0;
+  if ((p.flags & kFlagFS) && (mk & kFSNear)) fs_force<D, C, R>(p, b.tag, nb, rs, F);
+  if (p.flags & kFlagBody) {
+#pragma unroll
+    for (int m = 0; m < C; m++)
+#pragma unroll
+      for (int d = 0; d < D; d++) F[m][d] = F[m][d] + p.gvt[d] * p.mm[m] * rs[m];
+  }
+  if constexpr (NFF > 0) {
+    if (p.flags & kFlagFF) ff_force<D, C, NFF, R>(p, b.rho, node, nb, (mk & kFFClean) != 0, rs, F);
+  }
+  // common velocity (FlowUpdateUE, lbm_flow.F90:494-573)
+  R up[3];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    R num = (R)0, den = (R)0;
+#pragma unroll
+    for (int m = 0; m < C; m++) num = num + (j[m][d] + (R)0.5 * F[m][d]) * p.mmot[m];
+#pragma unroll
+    for (int m = 0; m < C; m++) den = den + rl[m] * p.mmot[m];
+    up[d] = num / den;
+  }
+  if (ex.f) {  // state export
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+#pragma unroll
+      for (int n = 0; n < Q; n++) ex.f[(size_t)(m * Q + n) * p.N + node] = (double)f[m][n];
+      ex.rho[(size_t)m * p.N + node] = (double)rl[m];
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        ex.F[(size_t)(m * D + d) * p.N + node] = (double)F[m][d];
+        ex.j[(size_t)(m * D + d) * p.N + node] = (double)j[m][d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) ex.up[(size_t)d * p.N + node] = (double)up[d];
+    return;
+  }
+  // collision (FlowCollisionD3/D2 lbm_flow.F90:960-1029, equilibrium
+  // DiscretizationEquilf lbm_discretization_d3q19.F90:265-305 / _d2q9.F90:168-205,
+  // RelaxationCollideSRT/MRT lbm_relaxation.F90:171-200)
+  R usqr = (R)0;
+#pragma unroll
+  for (int d = 0; d < D; d++) usqr = usqr + up[d] * up[d];
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    const R r = rl[m];
+    const R rcs = r * p.cs2[m];
+    R feq[Q], pref[Q];
+    sfor<0, Q>([&](auto NI) {
+      constexpr int n = decltype(NI)::value;
+      // prefactor: sum_d F_d (c_nd - u_d) / (rho c_s^2)
+      R s = (R)0;
+      sfor<0, D>([&](auto DI) {
+        constexpr int d = decltype(DI)::value;
+        s = s + F[m][d] * ((R)cdir<D>(n, d) - up[d]);
+      });
+      pref[n] = s / rcs;
+      if constexpr (n == 0) {
+        if constexpr (D == 3)
+          feq[0] = r * (p.dk[m] - usqr / (R)2.);
+        else
+          feq[0] = r * (p.feq0a[m] - (R)2. * usqr / (R)3.);
+      } else {
+        R ue = (R)0;
+        sfor<0, D>([&](auto DI) {
+          constexpr int d = decltype(DI)::value;
+          constexpr int c = cdir<D>(n, d);
+          if constexpr (c == 1) ue = ue + up[d];
+          if constexpr (c == -1) ue = ue - up[d];
+        });
+        feq[n] = (R)L::w[n] * r *
+                 (p.a15[m] + ue / p.cs2[m] + ue * ue / p.two_cs2_sq[m] - usqr / p.two_cs2[m]);
+      }
+    });
+    if (!(p.flags & kFlagMRT)) {
+#pragma unroll
+      for (int n = 0; n < Q; n++) {
+        const R bar = ((R)1. - (R)0.5 * pref[n]) * feq[n];
+        f[m][n] = f[m][n] - (f[m][n] - bar) / p.tau[m];
+      }
+    } else {
+      R dfi[Q];
+#pragma unroll
+      for (int n = 0; n < Q; n++) dfi[n] = f[m][n] - ((R)1. - (R)0.5 * pref[n]) * feq[n];
+      sfor<0, Q>([&](auto KI) {
+        constexpr int k = decltype(KI)::value;
+        R md = (R)0;
+        sfor<0, Q>([&](auto NI) {
+          constexpr int n = decltype(NI)::value;
+          constexpr int mv = L::M[k][n];
+          if constexpr (mv == 1) md = md + dfi[n];
+          else if constexpr (mv == -1) md = md - dfi[n];
+          else if constexpr (mv != 0) md = md + (R)mv * dfi[n];
+        });
+        const R t = p.rate[m][L::rate[k]] * md / (R)L::mmt[k];
+        sfor<0, Q>([&](auto NI) {
+          constexpr int n = decltype(NI)::value;
+          constexpr int mv = L::M[k][n];
+          if constexpr (mv == 1) f[m][n] = f[m][n] - t;
+          else if constexpr (mv == -1) f[m][n] = f[m][n] + t;
+          else if constexpr (mv != 0) f[m][n] = f[m][n] - t * (R)mv;
+        });
+      });
+    }
+#pragma unroll
+    for (int n = 0; n < Q; n++) {
+      f[m][n] = f[m][n] + pref[n] * feq[n];
+      st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, f[m][n], (R)L::w[n] * p.shift0[m]);
+    }
+  }
+}
+
+// FlowFiInit (lbm_flow.F90:923-934): f = (1 - pref/2) feq(rho_IC, u_IC) with
+// forces from the IC density; rho[] holds the IC density (NaN on solids).
+template <int D, int C, int NFF, class S, class R>
+__global__ void __launch_bounds__(256) k_fiinit(Params<R> p, Bufs<S, R> b, const R* __restrict__ u_ic) {
+  using L = Lat<D>;
+  constexpr int Q = L::Q;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= p.N) return;
+  const uint32_t mk = b.mask[node];
+  if (mk & kSelfSolid) {
+#pragma unroll
+    for (int m = 0; m < C; m++)
+#pragma unroll
+      for (int n = 0; n < Q; n++) st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, (R)0, (R)0);
+    return;
+  }
+  int x, y, z;
+  node_xyz(p, node, x, y, z);
+  Nbr nb;
+  make_nbr<D>(p, x, y, z, nb);
+  R rs[C];
+#pragma unroll
+  for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
+  R F[C][3];
+#pragma unroll
+  for (int m = 0; m < C; m++) F[m][0] = F[m][1] = F[m][2] = (R)0;
+  if ((p.flags & kFlagFS) && (mk & kFSNear)) fs_force<D, C, R>(p, b.tag, nb, rs, F);
+  if (p.flags & kFlagBody) {
+#pragma unroll
+    for (int m = 0; m < C; m++)
+#pragma unroll
+      for (int d = 0; d < D; d++) F[m][d] = F[m][d] + p.gvt[d] * p.mm[m] * rs[m];
+  }
+  if constexpr (NFF > 0) {
+    if (p.flags & kFlagFF) ff_force<D, C, NFF, R>(p, b.rho, node, nb, (mk & kFFClean) != 0, rs, F);
+  }
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R u[3];
+#pragma unroll
+    for (int d = 0; d < D; d++) u[d] = u_ic[((size_t)node * D + d) * C + m];
+    R usqr = (R)0;
+#pragma unroll
+    for (int d = 0; d < D; d++) usqr = usqr + u[d] * u[d];
+    const R r = rs[m];
+    sfor<0, Q>([&](auto NI) {
+      constexpr int n = decltype(NI)::value;
+      R s = (R)0;
+      sfor<0, D>([&](auto DI) {
+        constexpr int d = decltype(DI)::value;
+        s = s + F[m][d] * ((R)cdir<D>(n, d) - u[d]);
+      });
+      const R pref = s / (r * p.cs2[m]);
+      R feq;
+      if constexpr (n == 0) {
+        if constexpr (D == 3)
+          feq = r * (p.dk[m] - usqr / (R)2.);
+        else
+          feq = r * (p.feq0a[m] - (R)2. * usqr / (R)3.);
+      } else {
+        R ue = (R)0;
+        sfor<0, D>([&](auto DI) {
+          constexpr int d = decltype(DI)::value;
+          constexpr int c = cdir<D>(n, d);
+          if constexpr (c == 1) ue = ue + u[d];
+          if constexpr (c == -1) ue = ue - u[d];
+        });
+        feq = (R)L::w[n] * r * (p.a15[m] + ue / p.cs2[m] + ue * ue / p.two_cs2_sq[m] - usqr / p.two_cs2[m]);
+      }
+      st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, ((R)1. - (R)0.5 * pref) * feq, (R)L::w[n] * p.shift0[m]);
+    });
+  }
+}
+
+}  // namespace txg

@@ -1,0 +1,155 @@
+// txg_lattice.h -- D2Q9 / D3Q19 lattice constants and the fluid-fluid
+// stencil table, shared by host set-up code and the HIP kernels.
+//
+// Direction order, velocities, weights, MRT matrices and stencil weights are
+// those of the reference (lbm_discretization_d3q19.F90:20-232,
+// lbm_discretization_d2q9.F90:20-144); they have to be identical for the
+// populations to mean the same thing on both sides of the boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace txg {
+
+template <int D>
+struct Lat;
+
+template <>
+struct Lat<3> {
+  static constexpr int D = 3;
+  static constexpr int Q = 19;
+  static constexpr int NAX = 6;  // directions 1..NAX are the axis directions
+  static constexpr int cx[19] = {0, 1, 0, -1, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0};
+  static constexpr int cy[19] = {0, 0, 1, 0, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, -1, 1};
+  static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+  static constexpr int opp[19] = {0, 3, 4, 1, 2, 6, 5, 9, 10, 7, 8, 13, 14, 11, 12, 17, 18, 15, 16};
+  static constexpr double w[19] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                                   1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                                   1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                                   1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+  // M[moment][direction] (the transpose of the reference's mt_mrt)
+  static constexpr int M[19][19] = {
+      {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {-30, -11, -11, -11, -11, -11, -11, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8},
+      {12, -4, -4, -4, -4, -4, -4, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {0, 1, 0, -1, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0},
+      {0, -4, 0, 4, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0},
+      {0, 0, 1, 0, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, -1, 1},
+      {0, 0, -4, 0, 4, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, -1, 1},
+      {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1},
+      {0, 0, 0, 0, 0, -4, 4, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1},
+      {0, 2, -1, 2, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+      {0, -4, 2, -4, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+      {0, 0, 1, 0, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+      {0, 0, -2, 0, -2, 2, 2, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, 1, -1, -1, 1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, -1, -1, 1, 1}};
+  static constexpr int mmt[19] = {19, 2394, 252, 10, 40, 10, 40, 10, 40, 36, 72, 12, 24, 4, 4, 4, 8, 8, 8};
+  // which relaxation rate applies to each moment
+  // (lbm_discretization_d3q19.F90:240-262): 0 s_c, 1 s_e, 2 s_e2, 3 s_q,
+  // 4 s_nu, 5 s_pi, 6 s_m
+  static constexpr int rate[19] = {0, 1, 2, 0, 3, 0, 3, 0, 3, 4, 5, 4, 5, 4, 4, 4, 6, 6, 6};
+};
+
+template <>
+struct Lat<2> {
+  static constexpr int D = 2;
+  static constexpr int Q = 9;
+  static constexpr int NAX = 4;
+  static constexpr int cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+  static constexpr int cy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+  static constexpr int cz[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  static constexpr int opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+  static constexpr double w[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
+                                  1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+  static constexpr int M[9][9] = {{1, 1, 1, 1, 1, 1, 1, 1, 1},     {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+                                  {4, -2, -2, -2, -2, 1, 1, 1, 1}, {0, 1, 0, -1, 0, 1, -1, -1, 1},
+                                  {0, -2, 0, 2, 0, 1, -1, -1, 1},  {0, 0, 1, 0, -1, 1, 1, -1, -1},
+                                  {0, 0, -2, 0, 2, 1, 1, -1, -1},  {0, 1, -1, 1, -1, 0, 0, 0, 0},
+                                  {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+  static constexpr int mmt[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
+  // (lbm_discretization_d2q9.F90:154-164)
+  static constexpr int rate[9] = {0, 1, 2, 0, 3, 0, 3, 4, 4};
+};
+
+// ---------------------------------------------------------------------------
+// Fluid-fluid (psi-gradient) stencil: neighbour offsets in the order the
+// reference accumulates them (lbm_forcing.F90:84-939 for D3Q19,
+// :985-1285 for D2Q9).  The order fixes the FP64 rounding of the gradient
+// sums, so it is kept as a table rather than generated.
+constexpr int kFF3Count4 = 18, kFF3Count8 = 92;
+constexpr int kFF2Count4 = 8, kFF2Count8 = 24, kFF2Count10 = 36;
+
+constexpr int kFF3[92][3] = {
+    {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
+    {1, 1, 0},   {-1, 1, 0},  {1, -1, 0},  {-1, -1, 0}, {1, 0, 1},   {-1, 0, 1},
+    {1, 0, -1},  {-1, 0, -1}, {0, 1, 1},   {0, -1, 1},  {0, 1, -1},  {0, -1, -1},
+    {1, 1, 1},   {-1, 1, 1},  {-1, -1, 1}, {1, -1, 1},  {1, 1, -1},  {-1, 1, -1},
+    {-1, -1, -1}, {1, -1, -1}, {2, 0, 0},  {-2, 0, 0},  {0, 2, 0},   {0, -2, 0},
+    {0, 0, 2},   {0, 0, -2},  {2, 1, 0},   {-2, 1, 0},  {-2, -1, 0}, {2, -1, 0},
+    {2, 0, 1},   {-2, 0, 1},  {-2, 0, -1}, {2, 0, -1},  {1, 2, 0},   {-1, 2, 0},
+    {-1, -2, 0}, {1, -2, 0},  {0, 2, 1},   {0, -2, 1},  {0, 2, -1},  {0, -2, -1},
+    {1, 0, 2},   {0, 1, 2},   {-1, 0, 2},  {0, -1, 2},  {1, 0, -2},  {0, 1, -2},
+    {-1, 0, -2}, {0, -1, -2}, {2, 1, 1},   {-2, 1, 1},  {-2, -1, 1}, {2, -1, 1},
+    {2, 1, -1},  {-2, 1, -1}, {-2, -1, -1}, {2, -1, -1}, {1, 2, 1},  {-1, 2, 1},
+    {-1, -2, 1}, {1, -2, 1},  {1, 2, -1},  {-1, -2, -1}, {-1, 2, -1}, {1, -2, -1},
+    {1, 1, 2},   {-1, 1, 2},  {-1, -1, 2}, {1, -1, 2},  {1, 1, -2},  {-1, 1, -2},
+    {-1, -1, -2}, {1, -1, -2}, {2, 2, 0},  {-2, 2, 0},  {-2, -2, 0}, {2, -2, 0},
+    {2, 0, 2},   {-2, 0, 2},  {-2, 0, -2}, {2, 0, -2},  {0, 2, 2},   {0, -2, 2},
+    {0, -2, -2}, {0, 2, -2}};
+
+constexpr int kFF2[36][2] = {
+    {1, 0},  {-1, 0}, {0, 1},  {0, -1}, {1, 1},   {-1, 1},  {1, -1},  {-1, -1}, {2, 0},
+    {-2, 0}, {0, 2},  {0, -2}, {2, 1},  {2, -1},  {-2, 1},  {-2, -1}, {1, 2},   {1, -2},
+    {-1, 2}, {-1, -2}, {2, 2}, {-2, 2}, {2, -2},  {-2, -2}, {3, 0},   {-3, 0},  {0, 3},
+    {0, -3}, {3, 1},  {3, -1}, {-3, 1}, {-3, -1}, {1, 3},   {1, -3},  {-1, 3},  {-1, -3}};
+
+// Stencil weights ffw(k), indexed by |e|^2 (lbm_discretization_d3q19.F90:199-213,
+// lbm_discretization_d2q9.F90:120-142).  Zero where the isotropy has no such shell.
+inline void ff_weights(int dims, int iso, double ffw[16]) {
+  for (int i = 0; i < 16; i++) ffw[i] = 0.0;
+  if (dims == 3) {
+    if (iso == 4) {
+      ffw[1] = 1.0 / 6.0;
+      ffw[2] = 1.0 / 12.0;
+    } else if (iso == 8) {
+      ffw[1] = 4.0 / 45.0;
+      ffw[2] = 1.0 / 21.0;
+      ffw[3] = 2.0 / 105.0;
+      ffw[4] = 5.0 / 504.0;
+      ffw[5] = 1.0 / 315.0;
+      ffw[6] = 1.0 / 630.0;
+      ffw[8] = 1.0 / 5040.0;
+    }
+  } else {
+    if (iso == 4) {
+      ffw[1] = 1.0 / 3.0;
+      ffw[2] = 1.0 / 12.0;
+    } else if (iso == 8) {
+      ffw[1] = 4.0 / 21.0;
+      ffw[2] = 4.0 / 45.0;
+      ffw[4] = 1.0 / 60.0;
+      ffw[5] = 2.0 / 315.0;
+      ffw[8] = 1.0 / 5040.0;
+    } else if (iso == 10) {
+      ffw[1] = 262.0 / 1785.0;
+      ffw[2] = 93.0 / 1190.0;
+      ffw[4] = 7.0 / 340.0;
+      ffw[5] = 6.0 / 595.0;
+      ffw[8] = 9.0 / 9520.0;
+      ffw[9] = 2.0 / 5355.0;
+      ffw[10] = 1.0 / 7140.0;
+    }
+  }
+}
+
+inline int ff_count(int dims, int iso) {
+  if (dims == 3) return iso == 4 ? kFF3Count4 : (iso == 8 ? kFF3Count8 : 0);
+  return iso == 4 ? kFF2Count4 : (iso == 8 ? kFF2Count8 : (iso == 10 ? kFF2Count10 : 0));
+}
+
+}  // namespace txg

@@ -1,0 +1,285 @@
+"""Python binding of the MI355X Shan-Chen flow step (C ABI in include/taxila_lbm.h).
+
+The production host of this library is the Fortran-90 driver
+(taxila-lbm_amd/fortran); this ctypes binding serves the tests, the
+benchmark and the batched/served solve loops.  It loads the in-tree
+``build/libtaxila_lbm.so`` and raises if it is missing: there is no CPU
+fallback for the step.
+
+Host arrays use the reference's PETSc global-vector layout on one rank
+(owned nodes in natural order, DOF innermost), see include/taxila_lbm.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libtaxila_lbm.so")
+
+MAXC = 4
+MAXMIN = 16
+
+SRT, MRT = 0, 1
+FP64, FP32, FP32_FP16 = 0, 1, 2
+BC_NULL, BC_DENSITY = 0, 6
+FACES = ("xm", "xp", "ym", "yp", "zm", "zp")
+PETSC_VEC_CLASSID = 1211214  # PetscBinaryIO.py:113
+
+
+class Desc(C.Structure):
+    """Mirror of ``txg_desc``."""
+
+    _fields_ = [
+        ("ndims", C.c_int32),
+        ("ncomponents", C.c_int32),
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("periodic", C.c_int32 * 3),
+        ("isotropy_order", C.c_int32),
+        ("relaxation_mode", C.c_int32),
+        ("precision", C.c_int32),
+        ("nminerals", C.c_int32),
+        ("body_forces", C.c_int32),
+        ("bc_flags", C.c_int32 * 6),
+        ("reserved", C.c_int32),
+        ("gvt", C.c_double * 3),
+        ("mm", C.c_double * MAXC),
+        ("tau", C.c_double * MAXC),
+        ("s_c", C.c_double * MAXC),
+        ("s_e", C.c_double * MAXC),
+        ("s_e2", C.c_double * MAXC),
+        ("s_q", C.c_double * MAXC),
+        ("s_nu", C.c_double * MAXC),
+        ("s_pi", C.c_double * MAXC),
+        ("s_m", C.c_double * MAXC),
+        ("g", (C.c_double * MAXC) * MAXC),
+        ("gw", (C.c_double * MAXC) * MAXMIN),
+        ("bc_density", (C.c_double * MAXC) * 6),
+        ("rho0", C.c_double * MAXC),
+    ]
+
+    @property
+    def Q(self) -> int:
+        return 9 if self.ndims == 2 else 19
+
+    @property
+    def nnodes(self) -> int:
+        return self.nx * self.ny * (self.nz if self.ndims == 3 else 1)
+
+
+def make_desc(ndims=3, ncomponents=1, shape=(16, 16, 16), periodic=(1, 1, 1),
+              isotropy_order=4, relaxation_mode=SRT, precision=FP64,
+              mm=None, tau=None, s_c=None, s_e=None, s_e2=None, s_q=None,
+              s_nu=None, s_pi=None, s_m=None, g=None, gw=None, gvt=None,
+              bc=None, rho0=None) -> Desc:
+    """Build a descriptor with the reference defaults (lbm_options.F90:93-152,
+    lbm_relaxation.F90:58-80) overridden by the given values.
+
+    g is a CxC nested list of -g_ij; gw a list (per mineral) of per-component
+    -gw values; bc a dict face -> list of per-component densities for
+    -bc_density_<face>."""
+    d = Desc()
+    d.ndims = ndims
+    d.ncomponents = ncomponents
+    d.nx, d.ny = shape[0], shape[1]
+    d.nz = shape[2] if ndims == 3 else 1
+    for i in range(3):
+        d.periodic[i] = int(periodic[i]) if i < len(periodic) else 0
+    d.isotropy_order = isotropy_order
+    d.relaxation_mode = relaxation_mode
+    d.precision = precision
+    for m in range(MAXC):
+        d.mm[m] = 1.0
+        d.tau[m] = 1.0
+        for f in ("s_c", "s_e", "s_e2", "s_q", "s_nu", "s_pi", "s_m"):
+            getattr(d, f)[m] = 1.0
+        d.rho0[m] = 0.0
+    for name, val in (("mm", mm), ("tau", tau), ("s_c", s_c), ("s_e", s_e), ("s_e2", s_e2),
+                      ("s_q", s_q), ("s_nu", s_nu), ("s_pi", s_pi), ("s_m", s_m),
+                      ("rho0", rho0)):
+        if val is not None:
+            for m, v in enumerate(val):
+                getattr(d, name)[m] = float(v)
+    if g is not None:
+        for i, row in enumerate(g):
+            for j, v in enumerate(row):
+                d.g[i][j] = float(v)
+    if gw is not None:
+        d.nminerals = len(gw)
+        for k, row in enumerate(gw):
+            for m, v in enumerate(row):
+                d.gw[k][m] = float(v)
+    if gvt is not None:
+        d.body_forces = 1
+        for i, v in enumerate(gvt):
+            d.gvt[i] = float(v)
+    if bc:
+        for face, vals in bc.items():
+            f = FACES.index(face)
+            d.bc_flags[f] = BC_DENSITY
+            for m, v in enumerate(vals):
+                d.bc_density[f][m] = float(v)
+    return d
+
+
+# ---------------------------------------------------------------- PETSc I/O
+def read_petsc_vec(path_or_bytes) -> np.ndarray:
+    """Read a PETSc binary Vec: big-endian int32 classid 1211214, int32 n,
+    then n big-endian float64 (PetscBinaryIO.py:194-224; lbm_io.F90:71-117)."""
+    if isinstance(path_or_bytes, (bytes, bytearray)):
+        raw = bytes(path_or_bytes)
+    else:
+        with open(path_or_bytes, "rb") as f:
+            raw = f.read()
+    cid, n = struct.unpack(">ii", raw[:8])
+    if cid != PETSC_VEC_CLASSID:
+        raise ValueError("not a PETSc Vec file (classid %d)" % cid)
+    if len(raw) < 8 + 8 * n:
+        raise ValueError("truncated PETSc Vec file")
+    return np.frombuffer(raw, dtype=">f8", count=n, offset=8).astype(np.float64)
+
+
+def write_petsc_vec(path, vec) -> None:
+    """Write a PETSc binary Vec in the format read_petsc_vec reads."""
+    v = np.ascontiguousarray(vec, dtype=">f8").ravel()
+    with open(path, "wb") as f:
+        f.write(struct.pack(">ii", PETSC_VEC_CLASSID, v.size))
+        f.write(v.tobytes())
+
+
+# ---------------------------------------------------------------- C ABI
+_lib = None
+
+SYMBOLS = ("txg_desc_init", "txg_create", "txg_set_walls", "txg_set_state", "txg_set_fi", "txg_init",
+           "txg_step", "txg_get_fi", "txg_get_moments", "txg_diagnostics", "txg_reduce",
+           "txg_delta_norm", "txg_sync", "txg_last_error", "txg_destroy")
+
+
+class TxgError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load build/libtaxila_lbm.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TxgError("HIP library missing: %s (run taxila-lbm_amd/build.py)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.txg_desc_init.argtypes = [vp]
+        L.txg_create.argtypes = [vp, C.POINTER(vp)]
+        for name in ("txg_set_walls", "txg_set_fi", "txg_get_fi", "txg_reduce", "txg_delta_norm"):
+            getattr(L, name).argtypes = [vp, vp]
+        L.txg_set_state.argtypes = [vp, vp, vp]
+        L.txg_init.argtypes = [vp]
+        L.txg_sync.argtypes = [vp]
+        L.txg_destroy.argtypes = [vp]
+        L.txg_step.argtypes = [vp, C.c_int32]
+        L.txg_get_moments.argtypes = [vp, vp, vp, vp]
+        L.txg_diagnostics.argtypes = [vp, vp, vp, vp]
+        L.txg_last_error.argtypes = [vp, C.c_char_p, C.c_int32]
+        for name in SYMBOLS:
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Domain:
+    """One lattice domain on the current HIP device (a ``txg_handle``)."""
+
+    def __init__(self, desc: Desc):
+        self.desc = desc
+        h = C.c_void_p()
+        rc = lib().txg_create(C.byref(desc), C.byref(h))
+        if rc:
+            raise TxgError("txg_create failed (%d): %s" % (rc, self._err(None)))
+        self.h = h
+        self.C = desc.ncomponents
+        self.D = desc.ndims
+        self.Q = desc.Q
+        self.N = desc.nnodes
+
+    @staticmethod
+    def _err(h):
+        buf = C.create_string_buffer(512)
+        lib().txg_last_error(h, buf, 512)
+        return buf.value.decode()
+
+    def _check(self, rc, what):
+        if rc:
+            raise TxgError("%s failed (%d): %s" % (what, rc, self._err(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().txg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_walls(self, walls):
+        w = np.ascontiguousarray(walls, dtype=np.float64).ravel()
+        assert w.size == self.N
+        self._check(lib().txg_set_walls(self.h, _p(w)), "txg_set_walls")
+
+    def set_state(self, rho, u=None):
+        r = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+        assert r.size == self.N * self.C
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64).ravel()
+        self._check(lib().txg_set_state(self.h, _p(r), _p(uu)), "txg_set_state")
+
+    def set_fi(self, fi):
+        f = np.ascontiguousarray(fi, dtype=np.float64).ravel()
+        assert f.size == self.N * self.Q * self.C
+        self._check(lib().txg_set_fi(self.h, _p(f)), "txg_set_fi")
+
+    def init(self):
+        self._check(lib().txg_init(self.h), "txg_init")
+
+    def step(self, n=1):
+        self._check(lib().txg_step(self.h, int(n)), "txg_step")
+
+    def sync(self):
+        self._check(lib().txg_sync(self.h), "txg_sync")
+
+    def get_fi(self):
+        f = np.empty(self.N * self.Q * self.C)
+        self._check(lib().txg_get_fi(self.h, _p(f)), "txg_get_fi")
+        return f
+
+    def get_moments(self):
+        rho = np.empty(self.N * self.C)
+        flux = np.empty(self.N * self.C * self.D)
+        frc = np.empty(self.N * self.C * self.D)
+        self._check(lib().txg_get_moments(self.h, _p(rho), _p(flux), _p(frc)), "txg_get_moments")
+        return rho, flux, frc
+
+    def diagnostics(self):
+        rt = np.empty(self.N)
+        p = np.empty(self.N)
+        v = np.empty(self.N * self.D)
+        self._check(lib().txg_diagnostics(self.h, _p(rt), _p(p), _p(v)), "txg_diagnostics")
+        return rt, p, v
+
+    def reduce(self):
+        out = np.empty(self.C + self.C * self.D)
+        self._check(lib().txg_reduce(self.h, _p(out)), "txg_reduce")
+        return out
+
+    def delta_norm(self):
+        out = np.zeros(1)
+        self._check(lib().txg_delta_norm(self.h, _p(out)), "txg_delta_norm")
+        return float(out[0])

@@ -1,0 +1,84 @@
+"""HIP step vs the FP64 oracle and the reference's golden fixture (GPU).
+
+FP64 mode is required to match bit for bit (the kernels evaluate every
+expression in the reference's order without FMA contraction); the fluid
+nodes' populations, densities, common velocity and forces are compared.
+"""
+import numpy as np
+import pytest
+
+import cases
+import oracle
+import txg
+
+pytestmark = pytest.mark.gpu
+
+
+def fluid_mask(walls, per_node):
+    return np.repeat(walls == 0, per_node)
+
+
+def test_bubble_2d_fixture_bit_exact():
+    d, walls, rho, u = cases.bubble_2d_fixture()
+    dom = cases.run(txg.Domain, d, walls, rho, u, 100)
+    fi = dom.get_fi()
+    ref = cases.golden_vec("bubble_2D_fi001.dat.xz")
+    assert np.array_equal(fi, ref), "max diff %g" % np.abs(fi - ref).max()
+
+
+CASES = [
+    # (name, desc kwargs, walls builder, ic builder, steps)
+    ("d3_srt_iso4_bubble", dict(ndims=3, ncomponents=2, shape=(24, 20, 18), isotropy_order=4,
+                                g=[[0, 0.1], [0.1, 0]], tau=[1.0, 0.7]), None, "ball", 30),
+    ("d3_mrt_iso8_bubble", dict(ndims=3, ncomponents=2, shape=(20, 22, 24), isotropy_order=8,
+                                relaxation_mode=txg.MRT, g=[[0, 0.1], [0.1, 0]],
+                                s_c=[1.0, 1 / 0.6], s_e=[1.19, 1.19], s_e2=[1.4, 1.4], s_q=[1.2, 1.2],
+                                s_nu=[1.0, 1 / 0.6]), None, "ball", 30),
+    ("d3_porous_dirichlet", dict(ndims=3, ncomponents=2, shape=(20, 16, 18), periodic=(0, 0, 0),
+                                 isotropy_order=8, relaxation_mode=txg.MRT, g=[[0, 0.1], [0.1, 0]],
+                                 s_c=[1.0, 1 / 0.6], s_e=[1.19, 1.19], s_e2=[1.4, 1.4], s_q=[1.2, 1.2],
+                                 s_nu=[1.0, 1 / 0.6], gw=[[0.05, -0.05], [-0.03, 0.03]], gvt=[1e-5, 0, 0],
+                                 bc={"xm": [0.985, 0.03], "xp": [0.975, 0.03]}), "porous", "ball", 25),
+    ("d2_mrt_iso10", dict(ndims=2, ncomponents=2, shape=(48, 40), periodic=(1, 1), isotropy_order=10,
+                          relaxation_mode=txg.MRT, g=[[0, 0.05], [0.05, 0]], s_e=[0.1, 1.8],
+                          s_e2=[0.2, 1.8], s_q=[0.625, 1.8]), None, "ball", 40),
+    ("d2_walls_fs_body", dict(ndims=2, ncomponents=2, shape=(40, 36), periodic=(1, 0), isotropy_order=8,
+                              g=[[0, 0.12], [0.12, 0]], tau=[1.0, 0.8], gw=[[0.04, -0.04]], gvt=[2e-5, 0]),
+     "porous", "ball", 40),
+    ("d3_single_phase_body", dict(ndims=3, ncomponents=1, shape=(16, 18, 20), isotropy_order=0,
+                                  tau=[0.9], gvt=[1e-5, 0, 0]), "porous", "uniform", 30),
+]
+
+
+def build_case(name, kw, wmode, icmode):
+    d = txg.make_desc(**kw)
+    shape = (d.nx, d.ny, d.nz) if d.ndims == 3 else (d.nx, d.ny)
+    n = d.nnodes
+    if wmode == "porous":
+        walls = cases.random_walls(shape, 0.15, seed=7, minerals=(1.0, 2.0) if d.nminerals >= 2 else (1.0,))
+    else:
+        walls = np.zeros(n)
+    if icmode == "ball":
+        r = min(shape) // 4
+        rho = cases.ball_ic(shape, r, (0.03, 0.97), (0.97, 0.03))
+    else:
+        rho = np.ones(n * d.ncomponents)
+    u = np.zeros(n * d.ncomponents * d.ndims)
+    return d, walls, rho, u
+
+
+@pytest.mark.parametrize("name,kw,wmode,icmode,steps", CASES, ids=[c[0] for c in CASES])
+def test_fp64_bit_exact_vs_oracle(name, kw, wmode, icmode, steps):
+    d, walls, rho, u = build_case(name, kw, wmode, icmode)
+    g = cases.run(txg.Domain, d, walls, rho, u, steps)
+    o = cases.run(oracle.Oracle, d, walls, rho, u, steps)
+    fl = walls == 0
+    C, D, Q = d.ncomponents, d.ndims, d.Q
+    fg, fo = g.get_fi(), o.get_fi()
+    m = np.repeat(fl, Q * C)
+    assert np.array_equal(fg[m], fo[m]), "fi max diff %g" % np.abs(fg[m] - fo[m]).max()
+    rg, ug, Fg = g.get_moments()
+    ro, uo, Fo = o.get_moments()
+    assert np.array_equal(rg[np.repeat(fl, C)], ro[np.repeat(fl, C)])
+    assert np.array_equal(ug[np.repeat(fl, C * D)], uo[np.repeat(fl, C * D)])
+    assert np.array_equal(Fg[np.repeat(fl, C * D)], Fo[np.repeat(fl, C * D)])
