@@ -1,0 +1,152 @@
+#!/usr/bin/env python
+"""Throughput of the Shan-Chen flow step on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--cpu-seconds S]
+
+Prints one JSON line.  `value` is MLUPS (lattice nodes x steps / s / 1e6,
+every node counted) of the whole job with the state resident in HBM; one
+step is one timestep of the reference's flow loop (LBMRun2) over the whole
+lattice.  `roofline` prices the dominant kernel (the collide pass) at the
+algorithmic bytes per node-step B_alg = 2*Q*C*s_f + 1 (populations read
+once and written once, plus a 1-byte node tag) against MI355X's 8 TB/s,
+with durations from HIP events on the domain's stream over the timed
+region.  `cpu_baseline` times the FP64 CPU oracle (this repository's
+restatement of the reference step; the reference itself needs PETSc 3.6,
+which the GPU machine lacks) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "taxila-lbm_amd"))
+
+import numpy as np  # noqa: E402
+
+import txg  # noqa: E402
+import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+STORAGE_BYTES = {txg.FP64: 8, txg.FP32: 4, txg.FP32_FP16: 2}
+DTYPE = {txg.FP64: "f64", txg.FP32: "f32", txg.FP32_FP16: "f32 (f16 storage)"}
+
+
+def b_alg(desc):
+    return 2 * desc.Q * desc.ncomponents * STORAGE_BYTES[desc.precision] + 1
+
+
+def traffic_for(name):
+    """HBM bytes per collide launch from the committed rocprofv3 --pmc summary
+    (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data[name]["collide_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(name, seconds):
+    """FP64 oracle on the same workload, all (at most 16) host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # test infrastructure: the CPU baseline leg only
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    d, walls, rho, u, _ = workloads.workload(name)
+    o = oracle.Oracle(d)
+    o.set_walls(walls)
+    o.set_state(rho, u)
+    o.init()
+    o.step(1)  # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.step(1)
+        steps += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds or steps >= 200:
+            break
+    n = d.nnodes
+    return {"value": round(n * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": "%s, %d steps after 1 warm-up step, FP64 oracle (C, OpenMP)" % (name, steps)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="c1_bubble128_fp32_bgk", choices=sorted(workloads.WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="length of the CPU baseline sample (0 disables it)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    d, walls, rho, u, descr = workloads.workload(args.workload)
+    dom = txg.Domain(d)
+    dom.set_walls(walls)
+    dom.set_state(rho, u)
+    dom.init()
+    dom.step(args.warmup)
+    dom.sync()
+
+    dom.profile(True)
+    dom.sync()
+    t0 = time.perf_counter()
+    dom.step(args.steps)
+    dom.sync()
+    wall = time.perf_counter() - t0
+    prof = dom.profile_read()
+    dom.profile(False)
+
+    n = d.nnodes
+    ms_step = wall / args.steps * 1e3
+    mlups = n * args.steps / wall / 1e6
+    ba = b_alg(d)
+    coll_ms = prof["collide"][0] / max(1, prof["collide"][1])
+    dens_ms = prof["density"][0] / max(1, prof["density"][1])
+    achieved = ba * n / (coll_ms * 1e-3) / 1e9
+    step_gbs = ba * n / ((coll_ms + dens_ms) * 1e-3) / 1e9
+    traffic = traffic_for(args.workload)
+    masses = dom.reduce()[: d.ncomponents]
+    out = {
+        "metric": "MLUPS",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[d.precision],
+        "data": "synthetic",
+        "config": {"workload": args.workload, "description": descr, "lattice": "D3Q19" if d.ndims == 3 else "D2Q9",
+                   "nodes": n, "components": d.ncomponents,
+                   "collision": "MRT" if d.relaxation_mode == txg.MRT else "BGK",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "collide pass", "bytes_per_node_step": ba,
+                     "collide_ms": round(coll_ms, 5), "density_ms": round(dens_ms, 5),
+                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "mass": [float(x) for x in masses],
+    }
+    if args.cpu_seconds > 0:
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

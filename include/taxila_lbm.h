@@ -151,8 +151,18 @@ int txg_reduce(txg_handle* h, double* out);
 
 /* DistributionCalcDeltaNorm on rho (lbm_distribution_function.F90:809-833):
  * max |(rho_old - rho)/rho| over owned node-components with rho != 0, then
- * rho_old := rho.  The first call returns 1e99. */
+ * rho_old := rho.  rho_old starts at zero (VecSet, :173-174), so the first
+ * call returns 1. */
 int txg_delta_norm(txg_handle* h, double* norm);
+
+/* Kernel timing with HIP events on the handle's stream.  While enabled,
+ * every step kernel launch is bracketed by events; txg_profile_read returns
+ * the summed milliseconds and launch counts of the two step passes
+ * (ms[0], n[0]: collide pass; ms[1], n[1]: density pass) since the last
+ * enable, synchronising the stream.  No counterpart in the reference (its
+ * PETSc log stages, lbm_logging.F90:23-29, time the CPU phases). */
+int txg_profile_enable(txg_handle* h, int32_t on);
+int txg_profile_read(txg_handle* h, double* ms, int32_t* n);
 
 /* Wait for all queued work on the handle's stream. */
 int txg_sync(txg_handle* h);

@@ -138,6 +138,11 @@ struct txg_handle {
   std::vector<uint32_t> hmask;
   std::vector<uint8_t> htag;
   std::vector<int32_t> bb_node, bb_dir, fs_node, fs_dir, fs_min;
+  // event timing of the step passes (txg_profile_enable)
+  int prof = 0;
+  int ev_next = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_pairs[2];  // indices into ev_pool per pass
   int walls_set = 0, state_set = 0, fi_set = 0, initialized = 0;
   int local_state = 0;  // f[cur] holds post-stream populations (after init)
   std::string err;
@@ -612,19 +617,83 @@ int txg_init(txg_handle* h) {
   return h->hip(hipStreamSynchronize(h->stream), "init sync");
 }
 
+namespace {
+// record an event pair around one launch of pass `which` when profiling
+struct ProfScope {
+  txg_handle* h;
+  int which;
+  int a = -1;
+  ProfScope(txg_handle* hh, int w) : h(hh), which(w) {
+    if (!h->prof) return;
+    a = take();
+    (void)hipEventRecord(h->ev_pool[a], h->stream);
+  }
+  ~ProfScope() {
+    if (a < 0) return;
+    int b = take();
+    (void)hipEventRecord(h->ev_pool[b], h->stream);
+    h->ev_pairs[which].push_back({a, b});
+  }
+  int take() {
+    if (h->ev_next >= (int)h->ev_pool.size()) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      h->ev_pool.push_back(e);
+    }
+    return h->ev_next++;
+  }
+};
+}  // namespace
+
 int txg_step(txg_handle* h, int32_t nsteps) {
   if (!h || nsteps < 0) return TXG_ERR_ARG;
   if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
   for (int s = 0; s < nsteps; s++) {
     AnyBufs b = bufs(h, h->cur);
     int pulled = h->local_state ? 0 : 1;
-    int rc = h->hip(h->ks.collide(h->params(), b.ptr, pulled, pulled, nullptr, h->N, h->stream), "collide kernel");
+    int rc;
+    {
+      ProfScope ps(h, 0);
+      rc = h->hip(h->ks.collide(h->params(), b.ptr, pulled, pulled, nullptr, h->N, h->stream), "collide kernel");
+    }
     if (rc) return rc;
     h->cur = 1 - h->cur;
     h->local_state = 0;
     AnyBufs b2 = bufs(h, h->cur);
-    rc = h->hip(h->ks.density(h->params(), b2.ptr, 1, 1, h->N, h->stream), "density kernel");
+    {
+      ProfScope ps(h, 1);
+      rc = h->hip(h->ks.density(h->params(), b2.ptr, 1, 1, h->N, h->stream), "density kernel");
+    }
     if (rc) return rc;
+  }
+  return TXG_OK;
+}
+
+int txg_profile_enable(txg_handle* h, int32_t on) {
+  if (!h) return TXG_ERR_ARG;
+  int rc = h->hip(hipStreamSynchronize(h->stream), "profile sync");
+  if (rc) return rc;
+  h->ev_pairs[0].clear();
+  h->ev_pairs[1].clear();
+  h->ev_next = 0;
+  h->prof = on ? 1 : 0;
+  return TXG_OK;
+}
+
+int txg_profile_read(txg_handle* h, double* ms, int32_t* n) {
+  if (!h || !ms || !n) return TXG_ERR_ARG;
+  int rc = h->hip(hipStreamSynchronize(h->stream), "profile sync");
+  if (rc) return rc;
+  for (int w = 0; w < 2; w++) {
+    double tot = 0.0;
+    for (auto& pr : h->ev_pairs[w]) {
+      float t = 0.f;
+      rc = h->hip(hipEventElapsedTime(&t, h->ev_pool[pr.first], h->ev_pool[pr.second]), "event time");
+      if (rc) return rc;
+      tot += t;
+    }
+    ms[w] = tot;
+    n[w] = (int32_t)h->ev_pairs[w].size();
   }
   return TXG_OK;
 }
@@ -790,6 +859,7 @@ int txg_destroy(txg_handle* h) {
   (void)hipFree(h->ex);
   (void)hipFree(h->red);
   (void)hipFree(h->rho_old);
+  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return TXG_OK;

@@ -157,7 +157,8 @@ _lib = None
 
 SYMBOLS = ("txg_desc_init", "txg_create", "txg_set_walls", "txg_set_state", "txg_set_fi", "txg_init",
            "txg_step", "txg_get_fi", "txg_get_moments", "txg_diagnostics", "txg_reduce",
-           "txg_delta_norm", "txg_sync", "txg_last_error", "txg_destroy")
+           "txg_delta_norm", "txg_sync", "txg_last_error", "txg_destroy", "txg_profile_enable",
+           "txg_profile_read")
 
 
 class TxgError(RuntimeError):
@@ -184,6 +185,8 @@ def lib():
         L.txg_get_moments.argtypes = [vp, vp, vp, vp]
         L.txg_diagnostics.argtypes = [vp, vp, vp, vp]
         L.txg_last_error.argtypes = [vp, C.c_char_p, C.c_int32]
+        L.txg_profile_enable.argtypes = [vp, C.c_int32]
+        L.txg_profile_read.argtypes = [vp, vp, vp]
         for name in SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -283,3 +286,14 @@ class Domain:
         out = np.zeros(1)
         self._check(lib().txg_delta_norm(self.h, _p(out)), "txg_delta_norm")
         return float(out[0])
+
+    def profile(self, on=True):
+        """Bracket every step pass with HIP events (see txg_profile_enable)."""
+        self._check(lib().txg_profile_enable(self.h, 1 if on else 0), "txg_profile_enable")
+
+    def profile_read(self):
+        """{'collide': (ms_total, launches), 'density': (ms_total, launches)}"""
+        ms = np.zeros(2)
+        n = np.zeros(2, dtype=np.int32)
+        self._check(lib().txg_profile_read(self.h, _p(ms), _p(n)), "txg_profile_read")
+        return {"collide": (float(ms[0]), int(n[0])), "density": (float(ms[1]), int(n[1]))}
