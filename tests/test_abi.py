@@ -1,0 +1,79 @@
+"""The C ABI library loads and exports what include/taxila_lbm.h declares, and
+the descriptor layout seen by C, ctypes and the Fortran binding agree (CPU;
+no HIP call is made)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import txg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "taxila_lbm.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^int\s+(txg_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) >= 15
+    lib = ctypes.CDLL(txg.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(txg.SYMBOLS) == set(names)
+
+
+def test_library_has_gfx950_code_object():
+    """The library built here carries device code for gfx950 (the offload
+    bundle's target id is in the binary)."""
+    with open(txg.LIB_PATH, "rb") as f:
+        blob = f.read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_desc_layout_matches_c(tmp_path):
+    fields = [f[0] for f in txg.Desc._fields_]
+    src = ['#include <stddef.h>', '#include <stdio.h>', '#include "taxila_lbm.h"', "int main(void) {",
+           '  printf("size %zu\\n", sizeof(txg_desc));']
+    for f in fields:
+        src.append('  printf("%s %%zu\\n", offsetof(txg_desc, %s));' % (f, f))
+    src.append("  return 0; }")
+    c = tmp_path / "probe.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    vals = dict(line.split() for line in out if line)
+    assert int(vals["size"]) == ctypes.sizeof(txg.Desc)
+    for f in fields:
+        assert int(vals[f]) == getattr(txg.Desc, f).offset, f
+
+
+def test_desc_init_defaults():
+    d = txg.Desc()
+    assert txg.lib().txg_desc_init(ctypes.byref(d)) == 0
+    assert d.ndims == 3 and d.ncomponents == 1 and d.isotropy_order == 4
+    assert d.relaxation_mode == txg.SRT and d.precision == txg.FP64
+    assert list(d.tau) == [1.0] * 4 and list(d.s_nu) == [1.0] * 4 and list(d.mm) == [1.0] * 4
+
+
+def test_create_rejects_bad_descriptors_before_touching_the_device():
+    """Validation errors are reported with a message and no handle."""
+    for kw, msg in ((dict(ndims=4), "ndims"), (dict(ncomponents=0), "ncomponents"),
+                    (dict(isotropy_order=6), "isotropy"), (dict(ncomponents=3), "components"),
+                    (dict(tau=[0.4]), "tau")):
+        d = txg.make_desc(**{k: v for k, v in kw.items() if k not in ("ndims", "ncomponents")},
+                          ncomponents=kw.get("ncomponents", 1))
+        if "ndims" in kw:
+            d.ndims = kw["ndims"]
+        h = ctypes.c_void_p()
+        rc = txg.lib().txg_create(ctypes.byref(d), ctypes.byref(h))
+        assert rc in (1, 4) and not h.value
+        buf = ctypes.create_string_buffer(256)
+        txg.lib().txg_last_error(None, buf, 256)
+        assert msg in buf.value.decode()
