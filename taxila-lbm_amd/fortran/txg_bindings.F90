@@ -1,0 +1,141 @@
+! txg_bindings.F90 -- ISO_C_BINDING interface to the C ABI of
+! include/taxila_lbm.h (the MI355X flow step library).
+!
+! The derived type txg_desc is interoperable with the C struct.  C's
+! row-major 2-D members appear transposed here:
+!   C g[m][n]          -> desc%g(n, m)          (value of -g_<m><n>)
+!   C gw[k][m]         -> desc%gw(m, k)         (value of -gw_<mineral k>_<component m>)
+!   C bc_density[f][m] -> desc%bc_density(m, f) (face f = 1..6: xm xp ym yp zm zp)
+module txg_bindings
+  use iso_c_binding
+  implicit none
+
+  integer, parameter :: TXG_MAXC = 4, TXG_MAXMIN = 16
+  integer(c_int), parameter :: TXG_OK = 0, TXG_ERR_ARG = 1, TXG_ERR_HIP = 2, &
+       TXG_ERR_STATE = 3, TXG_ERR_UNSUPPORTED = 4, TXG_ERR_NOMEM = 5
+  integer(c_int32_t), parameter :: TXG_SRT = 0, TXG_MRT = 1
+  integer(c_int32_t), parameter :: TXG_FP64 = 0, TXG_FP32 = 1, TXG_FP32_FP16 = 2
+  integer(c_int32_t), parameter :: TXG_BC_NULL = 0, TXG_BC_DENSITY = 6
+
+  type, bind(C) :: txg_desc
+     integer(c_int32_t) :: ndims, ncomponents, nx, ny, nz
+     integer(c_int32_t) :: periodic(3)
+     integer(c_int32_t) :: isotropy_order, relaxation_mode, precision, nminerals, body_forces
+     integer(c_int32_t) :: bc_flags(6)
+     integer(c_int32_t) :: reserved
+     real(c_double) :: gvt(3)
+     real(c_double) :: mm(TXG_MAXC), tau(TXG_MAXC)
+     real(c_double) :: s_c(TXG_MAXC), s_e(TXG_MAXC), s_e2(TXG_MAXC), s_q(TXG_MAXC)
+     real(c_double) :: s_nu(TXG_MAXC), s_pi(TXG_MAXC), s_m(TXG_MAXC)
+     real(c_double) :: g(TXG_MAXC, TXG_MAXC)
+     real(c_double) :: gw(TXG_MAXC, TXG_MAXMIN)
+     real(c_double) :: bc_density(TXG_MAXC, 6)
+     real(c_double) :: rho0(TXG_MAXC)
+  end type txg_desc
+
+  interface
+     integer(c_int) function txg_desc_init(d) bind(C, name='txg_desc_init')
+       import :: c_int, txg_desc
+       type(txg_desc), intent(out) :: d
+     end function txg_desc_init
+
+     integer(c_int) function txg_create(d, h) bind(C, name='txg_create')
+       import :: c_int, c_ptr, txg_desc
+       type(txg_desc), intent(in) :: d
+       type(c_ptr), intent(out) :: h
+     end function txg_create
+
+     integer(c_int) function txg_set_walls(h, walls) bind(C, name='txg_set_walls')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: walls(*)
+     end function txg_set_walls
+
+     integer(c_int) function txg_set_state(h, rho, u) bind(C, name='txg_set_state')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: rho(*), u(*)
+     end function txg_set_state
+
+     integer(c_int) function txg_set_fi(h, fi) bind(C, name='txg_set_fi')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: fi(*)
+     end function txg_set_fi
+
+     integer(c_int) function txg_init(h) bind(C, name='txg_init')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function txg_init
+
+     integer(c_int) function txg_step(h, nsteps) bind(C, name='txg_step')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+     end function txg_step
+
+     integer(c_int) function txg_get_fi(h, fi) bind(C, name='txg_get_fi')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: fi(*)
+     end function txg_get_fi
+
+     integer(c_int) function txg_get_moments(h, rho, flux, forces) bind(C, name='txg_get_moments')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: rho(*), flux(*), forces(*)
+     end function txg_get_moments
+
+     integer(c_int) function txg_diagnostics(h, rhot, prs, velt) bind(C, name='txg_diagnostics')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: rhot(*), prs(*), velt(*)
+     end function txg_diagnostics
+
+     integer(c_int) function txg_reduce(h, out) bind(C, name='txg_reduce')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: out(*)
+     end function txg_reduce
+
+     integer(c_int) function txg_delta_norm(h, norm) bind(C, name='txg_delta_norm')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: norm
+     end function txg_delta_norm
+
+     integer(c_int) function txg_sync(h) bind(C, name='txg_sync')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function txg_sync
+
+     integer(c_int) function txg_last_error(h, buf, len) bind(C, name='txg_last_error')
+       import :: c_int, c_ptr, c_char, c_int32_t
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(out) :: buf(*)
+       integer(c_int32_t), value :: len
+     end function txg_last_error
+
+     integer(c_int) function txg_destroy(h) bind(C, name='txg_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function txg_destroy
+  end interface
+
+contains
+
+  ! text of the last error on h (or of the last failed create, h = c_null_ptr)
+  function txg_error_text(h) result(msg)
+    type(c_ptr), intent(in) :: h
+    character(len=512) :: msg
+    character(kind=c_char) :: buf(512)
+    integer(c_int) :: n, i
+    buf = c_null_char
+    n = txg_last_error(h, buf, 512_c_int32_t)
+    msg = ''
+    do i = 1, min(int(n), 511)
+       if (buf(i) == c_null_char) exit
+       msg(i:i) = buf(i)
+    end do
+  end function txg_error_text
+end module txg_bindings

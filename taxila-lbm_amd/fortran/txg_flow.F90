@@ -1,0 +1,365 @@
+! txg_flow.F90 -- the flow object of the Fortran driver, backed by the
+! MI355X step library through ISO_C_BINDING.
+!
+! Keeps the reference's flow-step procedure names (lbm_flow.F90:84-102) so
+! that the timestep loop of LBMRun2 (lbm.F90:279-406) reads the same:
+!   FlowCollision, FlowStream, FlowBounceback, FlowApplyBCs, FlowUpdateFlux.
+! On the GPU these phases are fused into two kernel passes per step, so
+! FlowCollision queues the whole step and the other four are no-ops that
+! remain as call sites.  Options are read under the reference's names
+! (lbm_options.F90, lbm_component.F90, lbm_relaxation.F90, lbm_mineral.F90,
+! lbm_flow.F90:1031-1309, lbm_info.F90:116-238); the one option this build
+! adds is -precision fp64|fp32|fp16 (default fp64, the reference's).
+module txg_flow
+  use iso_c_binding
+  use iso_fortran_env, only: int32, int64, real64
+  use txg_bindings
+  use txg_options
+  implicit none
+  private
+
+  type, public :: flow_type
+     type(txg_desc) :: desc
+     type(c_ptr) :: h = c_null_ptr
+     integer :: ndims = 3, ncomponents = 1, nx = 0, ny = 0, nz = 1, nnodes = 0, q = 19
+     character(len=64) :: names(TXG_MAXC)
+     character(len=64) :: mineral_names(TXG_MAXMIN)
+     logical :: io_fi = .false., io_rho = .true., io_velt = .true., io_prs = .true., io_rhot = .false.
+  end type flow_type
+
+  public :: FlowSetFromOptions, FlowSetUp, WallsSetValues, FlowInitializeState, FlowLoadFi, FlowInit, &
+       FlowCollision, FlowStream, FlowBounceback, FlowApplyBCs, FlowUpdateFlux, FlowGetFi, &
+       FlowDeltaNorm, FlowOutput, FlowDestroy, FlowCheck, write_petsc_vec, read_petsc_vec
+
+contains
+
+  subroutine FlowCheck(flow, rc, what)
+    type(flow_type), intent(in) :: flow
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= TXG_OK) then
+       write(*, '(A,A,A,I0,A,A)') 'ERROR in ', what, ' (', rc, '): ', trim(txg_error_text(flow%h))
+       error stop 1
+    end if
+  end subroutine FlowCheck
+
+  character(len=8) function itoa(i)
+    integer, intent(in) :: i
+    write(itoa, '(I0)') i
+  end function itoa
+
+  subroutine FlowSetFromOptions(flow, opts)
+    type(flow_type), intent(inout) :: flow
+    type(options_type), intent(in) :: opts
+    character(len=64) :: disc, prec, bname
+    character(len=3), parameter :: faces(6) = ['xm ', 'xp ', 'ym ', 'yp ', 'zm ', 'zp ']
+    logical :: flag, bval
+    integer :: m, n, k, f, nmax, mode, iso, nmin
+    integer(c_int) :: rc
+    double precision :: v, gv(3)
+
+    rc = txg_desc_init(flow%desc)
+    ! discretization (lbm_options.F90:293-317)
+    disc = 'D3Q19'
+    call OptionsGetString(opts, '-flow_discretization', disc, flag)
+    if (.not. flag) call OptionsGetString(opts, '-discretization', disc, flag)
+    select case (lower(trim(disc)))
+    case ('d3q19')
+       flow%ndims = 3
+       flow%q = 19
+    case ('d2q9')
+       flow%ndims = 2
+       flow%q = 9
+    case default
+       write(*, *) 'ERROR: Invalid Discretization ', trim(disc)
+       error stop 1
+    end select
+    flow%desc%ndims = flow%ndims
+    ! grid (lbm_info.F90:136-161)
+    call OptionsGetInt(opts, '-NX', flow%nx, flag)
+    call OptionsGetInt(opts, '-NY', flow%ny, flag)
+    flow%nz = 1
+    if (flow%ndims == 3) call OptionsGetInt(opts, '-NZ', flow%nz, flag)
+    flow%desc%nx = flow%nx
+    flow%desc%ny = flow%ny
+    flow%desc%nz = flow%nz
+    flow%nnodes = flow%nx * flow%ny * flow%nz
+    bval = .false.
+    call OptionsGetBool(opts, '-bc_periodic_x', bval, flag)
+    flow%desc%periodic(1) = merge(1, 0, bval)
+    bval = .false.
+    call OptionsGetBool(opts, '-bc_periodic_y', bval, flag)
+    flow%desc%periodic(2) = merge(1, 0, bval)
+    bval = .false.
+    if (flow%ndims == 3) call OptionsGetBool(opts, '-bc_periodic_z', bval, flag)
+    flow%desc%periodic(3) = merge(1, 0, bval)
+    ! physics switches (lbm_options.F90:269-291)
+    mode = 0
+    call OptionsGetInt(opts, '-flow_relaxation_mode', mode, flag)
+    flow%desc%relaxation_mode = mode
+    iso = 4
+    call OptionsGetInt(opts, '-isotropy_order', iso, flag)
+    if (.not. flag) call OptionsGetInt(opts, '-derivative_order', iso, flag)
+    flow%desc%isotropy_order = iso
+    n = 1
+    call OptionsGetInt(opts, '-ncomponents', n, flag)
+    flow%ncomponents = n
+    flow%desc%ncomponents = n
+    prec = 'fp64'
+    call OptionsGetString(opts, '-precision', prec, flag)
+    select case (lower(trim(prec)))
+    case ('fp64')
+       flow%desc%precision = TXG_FP64
+    case ('fp32')
+       flow%desc%precision = TXG_FP32
+    case ('fp16')
+       flow%desc%precision = TXG_FP32_FP16
+    case default
+       write(*, *) 'ERROR: -precision must be fp64, fp32 or fp16'
+       error stop 1
+    end select
+    ! components (lbm_component.F90:131-176, lbm_relaxation.F90:117-151)
+    do m = 1, flow%ncomponents
+       flow%names(m) = 'component' // trim(itoa(m))
+       call OptionsGetString(opts, '-' // trim(flow%names(m)) // '_name', flow%names(m), flag)
+       call OptionsGetReal(opts, '-mm_' // trim(flow%names(m)), flow%desc%mm(m), flag)
+       if (mode == TXG_SRT) then
+          call OptionsGetReal(opts, '-tau_' // trim(flow%names(m)), flow%desc%tau(m), flag)
+       else
+          call OptionsGetReal(opts, '-s_c_' // trim(flow%names(m)), flow%desc%s_c(m), flag)
+          call OptionsGetReal(opts, '-s_e_' // trim(flow%names(m)), flow%desc%s_e(m), flag)
+          call OptionsGetReal(opts, '-s_e2_' // trim(flow%names(m)), flow%desc%s_e2(m), flag)
+          call OptionsGetReal(opts, '-s_q_' // trim(flow%names(m)), flow%desc%s_q(m), flag)
+          call OptionsGetReal(opts, '-s_nu_' // trim(flow%names(m)), flow%desc%s_nu(m), flag)
+          call OptionsGetReal(opts, '-s_pi_' // trim(flow%names(m)), flow%desc%s_pi(m), flag)
+          call OptionsGetReal(opts, '-s_m_' // trim(flow%names(m)), flow%desc%s_m(m), flag)
+       end if
+       flow%desc%rho0(m) = 0.5d0
+       call OptionsGetReal(opts, '-rho0_' // trim(flow%names(m)), flow%desc%rho0(m), flag)
+    end do
+    ! interaction coefficients -g_<m><n> (lbm_component.F90:165-173)
+    do m = 1, flow%ncomponents
+       do n = 1, flow%ncomponents
+          v = 0.d0
+          call OptionsGetReal(opts, '-g_' // trim(itoa(m)) // trim(itoa(n)), v, flag)
+          flow%desc%g(n, m) = v
+       end do
+    end do
+    ! body force -gvt (lbm_flow.F90:196-206)
+    gv = 0.d0
+    nmax = flow%ndims
+    call OptionsGetRealArray(opts, '-gvt', gv, nmax, flag)
+    if (flag) then
+       flow%desc%body_forces = 1
+       flow%desc%gvt(1:flow%ndims) = gv(1:flow%ndims)
+    end if
+    ! minerals (lbm_walls.F90:113-125, lbm_mineral.F90:103-134)
+    nmin = 1
+    call OptionsGetInt(opts, '-nminerals', nmin, flag)
+    if (nmin > TXG_MAXMIN) then
+       write(*, *) 'ERROR: at most ', TXG_MAXMIN, ' minerals'
+       error stop 1
+    end if
+    flow%desc%nminerals = nmin
+    do k = 1, nmin
+       flow%mineral_names(k) = 'mineral' // trim(itoa(k))
+       call OptionsGetString(opts, '-' // trim(flow%mineral_names(k)) // '_name', flow%mineral_names(k), flag)
+       do m = 1, flow%ncomponents
+          call OptionsGetReal(opts, '-gw_' // trim(flow%mineral_names(k)) // '_' // trim(flow%names(m)), &
+               flow%desc%gw(m, k), flag)
+       end do
+    end do
+    ! faces: periodic, or density (FlowParseBC, lbm_flow.F90:1031-1093)
+    do f = 1, 2 * flow%ndims
+       bname = trim(faces(f))
+       bval = .false.
+       call OptionsGetBool(opts, '-bc_density_' // trim(bname), bval, flag)
+       if (bval) then
+          flow%desc%bc_flags(f) = TXG_BC_DENSITY
+          if (flow%ncomponents == 1) then
+             flow%desc%bc_density(1, f) = -1.d0
+             call OptionsGetReal(opts, '-bc_density_' // trim(bname) // '_value', flow%desc%bc_density(1, f), flag)
+          else
+             do m = 1, flow%ncomponents
+                flow%desc%bc_density(m, f) = -1.d0
+                call OptionsGetReal(opts, '-bc_density_' // trim(bname) // '_' // trim(flow%names(m)), &
+                     flow%desc%bc_density(m, f), flag)
+             end do
+          end if
+       end if
+    end do
+    ! outputs (lbm_flow.F90:267-279)
+    call OptionsGetBool(opts, '-output_flow_fi', flow%io_fi, flag)
+    call OptionsGetBool(opts, '-output_rho', flow%io_rho, flag)
+    call OptionsGetBool(opts, '-output_velocity', flow%io_velt, flag)
+    call OptionsGetBool(opts, '-output_rhot', flow%io_rhot, flag)
+    call OptionsGetBool(opts, '-output_pressure', flow%io_prs, flag)
+  end subroutine FlowSetFromOptions
+
+  ! LBMSetUp's flow part (lbm.F90:140-175): device state for the domain
+  subroutine FlowSetUp(flow)
+    type(flow_type), intent(inout) :: flow
+    call FlowCheck(flow, txg_create(flow%desc, flow%h), 'txg_create')
+  end subroutine FlowSetUp
+
+  ! WallsSetValues / WallsSetGhostNodes / WallsCommunicate (lbm_walls.F90:151-244)
+  subroutine WallsSetValues(flow, walls)
+    type(flow_type), intent(inout) :: flow
+    real(real64), intent(in) :: walls(:)
+    call FlowCheck(flow, txg_set_walls(flow%h, walls), 'txg_set_walls')
+  end subroutine WallsSetValues
+
+  ! what the initialize_state callback produced (rho(C,node), u(C,D,node))
+  subroutine FlowInitializeState(flow, rho, u)
+    type(flow_type), intent(inout) :: flow
+    real(real64), intent(in) :: rho(:), u(:)
+    call FlowCheck(flow, txg_set_state(flow%h, rho, u), 'txg_set_state')
+  end subroutine FlowInitializeState
+
+  ! restart / -ic_file path (lbm.F90:482-544)
+  subroutine FlowLoadFi(flow, fi)
+    type(flow_type), intent(inout) :: flow
+    real(real64), intent(in) :: fi(:)
+    call FlowCheck(flow, txg_set_fi(flow%h, fi), 'txg_set_fi')
+  end subroutine FlowLoadFi
+
+  ! FlowFiInit + FlowUpdateMoments (lbm.F90:209-238)
+  subroutine FlowInit(flow)
+    type(flow_type), intent(inout) :: flow
+    call FlowCheck(flow, txg_init(flow%h), 'txg_init')
+  end subroutine FlowInit
+
+  ! one whole timestep, queued on the GPU (FlowCollision through FlowUpdateFlux)
+  subroutine FlowCollision(flow)
+    type(flow_type), intent(inout) :: flow
+    call FlowCheck(flow, txg_step(flow%h, 1_c_int32_t), 'txg_step')
+  end subroutine FlowCollision
+
+  subroutine FlowStream(flow)
+    type(flow_type), intent(inout) :: flow
+  end subroutine FlowStream
+
+  subroutine FlowBounceback(flow)
+    type(flow_type), intent(inout) :: flow
+  end subroutine FlowBounceback
+
+  subroutine FlowApplyBCs(flow)
+    type(flow_type), intent(inout) :: flow
+  end subroutine FlowApplyBCs
+
+  subroutine FlowUpdateFlux(flow)
+    type(flow_type), intent(inout) :: flow
+  end subroutine FlowUpdateFlux
+
+  subroutine FlowGetFi(flow, fi)
+    type(flow_type), intent(inout) :: flow
+    real(real64), intent(out) :: fi(:)
+    call FlowCheck(flow, txg_get_fi(flow%h, fi), 'txg_get_fi')
+  end subroutine FlowGetFi
+
+  ! DistributionCalcDeltaNorm on rho (lbm_distribution_function.F90:809-833)
+  subroutine FlowDeltaNorm(flow, norm)
+    type(flow_type), intent(inout) :: flow
+    real(real64), intent(out) :: norm
+    call FlowCheck(flow, txg_delta_norm(flow%h, norm), 'txg_delta_norm')
+  end subroutine FlowDeltaNorm
+
+  ! FlowUpdateDiagnostics + FlowOutputDiagnostics (lbm_flow.F90:576-708):
+  ! <prefix><name><NNN>.dat in PETSc binary (IOView, lbm_io.F90:71-93)
+  subroutine FlowOutput(flow, prefix, counter)
+    type(flow_type), intent(inout) :: flow
+    character(len=*), intent(in) :: prefix
+    integer, intent(in) :: counter
+    real(real64), allocatable :: a(:), b(:), c(:)
+    character(len=3) :: cnt
+    write(cnt, '(I3.3)') counter
+    if (flow%io_fi) then
+       allocate(a(flow%nnodes * flow%q * flow%ncomponents))
+       call FlowGetFi(flow, a)
+       call write_petsc_vec(trim(prefix) // 'fi' // cnt // '.dat', a)
+       deallocate(a)
+    end if
+    if (flow%io_rho) then
+       allocate(a(flow%nnodes * flow%ncomponents), b(flow%nnodes * flow%ncomponents * flow%ndims), &
+            c(flow%nnodes * flow%ncomponents * flow%ndims))
+       call FlowCheck(flow, txg_get_moments(flow%h, a, b, c), 'txg_get_moments')
+       call write_petsc_vec(trim(prefix) // 'rho' // cnt // '.dat', a)
+       deallocate(a, b, c)
+    end if
+    if (flow%io_velt .or. flow%io_prs .or. flow%io_rhot) then
+       allocate(a(flow%nnodes), b(flow%nnodes), c(flow%nnodes * flow%ndims))
+       call FlowCheck(flow, txg_diagnostics(flow%h, a, b, c), 'txg_diagnostics')
+       if (flow%io_velt) call write_petsc_vec(trim(prefix) // 'u' // cnt // '.dat', c)
+       if (flow%io_rhot) call write_petsc_vec(trim(prefix) // 'rhot' // cnt // '.dat', a)
+       if (flow%io_prs) call write_petsc_vec(trim(prefix) // 'prs' // cnt // '.dat', b)
+       deallocate(a, b, c)
+    end if
+  end subroutine FlowOutput
+
+  subroutine FlowDestroy(flow)
+    type(flow_type), intent(inout) :: flow
+    integer(c_int) :: rc
+    if (c_associated(flow%h)) rc = txg_destroy(flow%h)
+    flow%h = c_null_ptr
+  end subroutine FlowDestroy
+
+  ! ------------------------------------------------------------ PETSc binary
+  ! big-endian: int32 classid 1211214, int32 n, n float64 (lbm_io.F90:71-117)
+  elemental integer(int64) function bswap64(x)
+    integer(int64), intent(in) :: x
+    integer :: k
+    bswap64 = 0_int64
+    do k = 0, 7
+       bswap64 = ior(bswap64, ishft(iand(ishft(x, -8 * k), 255_int64), 8 * (7 - k)))
+    end do
+  end function bswap64
+
+  elemental integer(int32) function bswap32(x)
+    integer(int32), intent(in) :: x
+    integer(int64) :: y
+    integer :: k
+    y = iand(int(x, int64), 4294967295_int64)
+    bswap32 = 0
+    do k = 0, 3
+       bswap32 = ior(bswap32, int(ishft(iand(ishft(y, -8 * k), 255_int64), 8 * (3 - k)), int32))
+    end do
+  end function bswap32
+
+  subroutine write_petsc_vec(path, v)
+    character(len=*), intent(in) :: path
+    real(real64), intent(in) :: v(:)
+    integer :: u
+    integer(int64), allocatable :: w(:)
+    allocate(w(size(v)))
+    w = bswap64(transfer(v, 0_int64, size(v)))
+    open(newunit=u, file=trim(path), access='stream', form='unformatted', status='replace', action='write')
+    write(u) bswap32(1211214_int32), bswap32(int(size(v), int32))
+    write(u) w
+    close(u)
+  end subroutine write_petsc_vec
+
+  subroutine read_petsc_vec(path, v, n)
+    character(len=*), intent(in) :: path
+    real(real64), allocatable, intent(out) :: v(:)
+    integer, intent(out) :: n
+    integer :: u, ios
+    integer(int32) :: hdr(2)
+    integer(int64), allocatable :: w(:)
+    open(newunit=u, file=trim(path), access='stream', form='unformatted', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+       write(*, *) 'ERROR: cannot open ', trim(path)
+       error stop 1
+    end if
+    read(u) hdr
+    hdr = bswap32(hdr)
+    if (hdr(1) /= 1211214) then
+       write(*, *) 'ERROR: not a PETSc Vec file: ', trim(path)
+       error stop 1
+    end if
+    n = hdr(2)
+    allocate(w(n), v(n))
+    read(u) w
+    close(u)
+    v = transfer(bswap64(w), 0.0_real64, n)
+  end subroutine read_petsc_vec
+end module txg_flow

@@ -1,0 +1,250 @@
+! txg_options.F90 -- the options database of the Fortran driver.
+!
+! Reads PETSc-style options files ("-key [value]" tokens, '#' starts a
+! comment; PetscOptionsInsertFile semantics as used by main.F90:46) and the
+! command line, and answers OptionsGet* queries under the reference's option
+! names (lbm_options.F90:416-548).  Later definitions override earlier ones;
+! keys compare case-insensitively.
+module txg_options
+  implicit none
+  private
+
+  integer, parameter :: OPT_MAX = 1024, KEYLEN = 64, VALLEN = 256
+
+  type, public :: options_type
+     integer :: n = 0
+     character(len=KEYLEN) :: key(OPT_MAX)
+     character(len=VALLEN) :: val(OPT_MAX)
+     logical :: hasval(OPT_MAX)
+  end type options_type
+
+  public :: OptionsLoadFile, OptionsLoadArgs, OptionsSet, OptionsHas, OptionsGetInt, &
+       OptionsGetReal, OptionsGetBool, OptionsGetString, OptionsGetRealArray, lower
+
+contains
+
+  function lower(s) result(r)
+    character(len=*), intent(in) :: s
+    character(len=len(s)) :: r
+    integer :: i, c
+    r = s
+    do i = 1, len(s)
+       c = iachar(s(i:i))
+       if (c >= 65 .and. c <= 90) r(i:i) = achar(c + 32)
+    end do
+  end function lower
+
+  subroutine OptionsSet(opts, key, val, hasval)
+    type(options_type), intent(inout) :: opts
+    character(len=*), intent(in) :: key, val
+    logical, intent(in) :: hasval
+    integer :: i
+    do i = 1, opts%n
+       if (lower(trim(opts%key(i))) == lower(trim(key))) then
+          opts%val(i) = val
+          opts%hasval(i) = hasval
+          return
+       end if
+    end do
+    if (opts%n >= OPT_MAX) stop 'txg_options: too many options'
+    opts%n = opts%n + 1
+    opts%key(opts%n) = key
+    opts%val(opts%n) = val
+    opts%hasval(opts%n) = hasval
+  end subroutine OptionsSet
+
+  ! a token that starts with '-' is a key unless it is a negative number
+  logical function is_key(tok)
+    character(len=*), intent(in) :: tok
+    is_key = .false.
+    if (len_trim(tok) < 2) return
+    if (tok(1:1) /= '-') return
+    is_key = index('0123456789.', tok(2:2)) == 0
+  end function is_key
+
+  subroutine add_tokens(opts, ntok, toks)
+    type(options_type), intent(inout) :: opts
+    integer, intent(in) :: ntok
+    character(len=*), intent(in) :: toks(:)
+    integer :: i
+    i = 1
+    do while (i <= ntok)
+       if (is_key(toks(i))) then
+          if (i < ntok) then
+             if (.not. is_key(toks(i + 1))) then
+                call OptionsSet(opts, trim(toks(i)), trim(toks(i + 1)), .true.)
+                i = i + 2
+                cycle
+             end if
+          end if
+          call OptionsSet(opts, trim(toks(i)), '', .false.)
+       end if
+       i = i + 1
+    end do
+  end subroutine add_tokens
+
+  subroutine OptionsLoadFile(opts, path, ierr)
+    type(options_type), intent(inout) :: opts
+    character(len=*), intent(in) :: path
+    integer, intent(out) :: ierr
+    character(len=1024) :: line
+    character(len=VALLEN) :: toks(64)
+    integer :: u, ios, p, ntok, s
+    open(newunit=u, file=trim(path), status='old', action='read', iostat=ierr)
+    if (ierr /= 0) return
+    do
+       read(u, '(A)', iostat=ios) line
+       if (ios /= 0) exit
+       p = index(line, '#')
+       if (p > 0) line(p:) = ' '
+       ntok = 0
+       s = 1
+       do while (s <= len_trim(line))
+          do while (s <= len_trim(line) .and. (line(s:s) == ' ' .or. line(s:s) == achar(9)))
+             s = s + 1
+          end do
+          if (s > len_trim(line)) exit
+          p = s
+          do while (p <= len_trim(line) .and. line(p:p) /= ' ' .and. line(p:p) /= achar(9))
+             p = p + 1
+          end do
+          ntok = ntok + 1
+          toks(ntok) = line(s:p - 1)
+          s = p
+       end do
+       if (ntok > 0) call add_tokens(opts, ntok, toks)
+    end do
+    close(u)
+    ierr = 0
+  end subroutine OptionsLoadFile
+
+  subroutine OptionsLoadArgs(opts, first)
+    type(options_type), intent(inout) :: opts
+    integer, intent(in) :: first
+    character(len=VALLEN) :: toks(256)
+    integer :: i, n
+    n = 0
+    do i = first, min(command_argument_count(), first + 255)
+       n = n + 1
+       call get_command_argument(i, toks(n))
+    end do
+    if (n > 0) call add_tokens(opts, n, toks)
+  end subroutine OptionsLoadArgs
+
+  integer function find(opts, key)
+    type(options_type), intent(in) :: opts
+    character(len=*), intent(in) :: key
+    integer :: i
+    find = 0
+    do i = 1, opts%n
+       if (lower(trim(opts%key(i))) == lower(trim(key))) then
+          find = i
+          return
+       end if
+    end do
+  end function find
+
+  logical function OptionsHas(opts, key)
+    type(options_type), intent(in) :: opts
+    character(len=*), intent(in) :: key
+    OptionsHas = find(opts, key) > 0
+  end function OptionsHas
+
+  subroutine OptionsGetInt(opts, key, v, flag)
+    type(options_type), intent(in) :: opts
+    character(len=*), intent(in) :: key
+    integer, intent(inout) :: v
+    logical, intent(out) :: flag
+    integer :: i, ios, t
+    i = find(opts, key)
+    flag = .false.
+    if (i == 0) return
+    if (.not. opts%hasval(i)) return
+    read(opts%val(i), *, iostat=ios) t
+    if (ios /= 0) then
+       write(*, *) 'txg_options: bad integer for ', trim(key), ': ', trim(opts%val(i))
+       error stop 1
+    end if
+    v = t
+    flag = .true.
+  end subroutine OptionsGetInt
+
+  subroutine OptionsGetReal(opts, key, v, flag)
+    type(options_type), intent(in) :: opts
+    character(len=*), intent(in) :: key
+    double precision, intent(inout) :: v
+    logical, intent(out) :: flag
+    integer :: i, ios
+    double precision :: t
+    i = find(opts, key)
+    flag = .false.
+    if (i == 0) return
+    if (.not. opts%hasval(i)) return
+    read(opts%val(i), *, iostat=ios) t
+    if (ios /= 0) then
+       write(*, *) 'txg_options: bad real for ', trim(key), ': ', trim(opts%val(i))
+       error stop 1
+    end if
+    v = t
+    flag = .true.
+  end subroutine OptionsGetReal
+
+  subroutine OptionsGetRealArray(opts, key, v, n, flag)
+    type(options_type), intent(in) :: opts
+    character(len=*), intent(in) :: key
+    double precision, intent(inout) :: v(:)
+    integer, intent(inout) :: n   ! in: capacity, out: values read
+    logical, intent(out) :: flag
+    integer :: i, ios, k, cnt, p
+    character(len=VALLEN) :: s
+    i = find(opts, key)
+    flag = .false.
+    if (i == 0) return
+    if (.not. opts%hasval(i)) return
+    s = opts%val(i)
+    cnt = 1
+    do p = 1, len_trim(s)
+       if (s(p:p) == ',') cnt = cnt + 1
+    end do
+    cnt = min(cnt, n)
+    read(s, *, iostat=ios) (v(k), k = 1, cnt)
+    if (ios /= 0) then
+       write(*, *) 'txg_options: bad real array for ', trim(key), ': ', trim(s)
+       error stop 1
+    end if
+    n = cnt
+    flag = .true.
+  end subroutine OptionsGetRealArray
+
+  subroutine OptionsGetBool(opts, key, v, flag)
+    type(options_type), intent(in) :: opts
+    character(len=*), intent(in) :: key
+    logical, intent(inout) :: v
+    logical, intent(out) :: flag
+    integer :: i
+    character(len=VALLEN) :: s
+    i = find(opts, key)
+    flag = i > 0
+    if (i == 0) return
+    if (.not. opts%hasval(i)) then
+       v = .true.
+       return
+    end if
+    s = lower(trim(opts%val(i)))
+    v = .not. (s == 'false' .or. s == '0' .or. s == 'no' .or. s == 'off')
+  end subroutine OptionsGetBool
+
+  subroutine OptionsGetString(opts, key, v, flag)
+    type(options_type), intent(in) :: opts
+    character(len=*), intent(in) :: key
+    character(len=*), intent(inout) :: v
+    logical, intent(out) :: flag
+    integer :: i
+    i = find(opts, key)
+    flag = .false.
+    if (i == 0) return
+    if (.not. opts%hasval(i)) return
+    v = opts%val(i)
+    flag = .true.
+  end subroutine OptionsGetString
+end module txg_options

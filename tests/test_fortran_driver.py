@@ -1,0 +1,55 @@
+"""The Fortran-90 driver (taxila-lbm_amd/fortran) over the C ABI.
+
+CPU: option parsing and error exits.  GPU: the reference's tests/bubble_2D
+regression case run end to end through the Fortran driver (options file ->
+ISO_C_BINDING -> HIP step -> PETSc-binary output) reproduces the reference's
+fi001.dat bit for bit, as the reference's `make test` compares it
+(check_solution.py, L-inf <= 1e-5; here exactly).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+import txg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "taxila-lbm_amd", "build", "taxila_gpu")
+OPTS = os.path.join(ROOT, "taxila-lbm_amd", "fortran", "examples", "bubble_2D.opts")
+
+needs_driver = pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+
+
+@needs_driver
+def test_bad_discretization_is_rejected(tmp_path):
+    r = subprocess.run([EXE, OPTS, "-discretization", "d4q99"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode != 0
+    assert "Invalid Discretization" in r.stdout + r.stderr
+
+
+@needs_driver
+def test_missing_options_file(tmp_path):
+    r = subprocess.run([EXE, str(tmp_path / "nope.opts")], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "cannot read options file" in r.stdout + r.stderr
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_bubble_2d_through_fortran_driver(tmp_path):
+    out = tmp_path / "test_solution"
+    out.mkdir()
+    r = subprocess.run([EXE, OPTS, "-output_file_prefix", str(out) + "/"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    fi = txg.read_petsc_vec(str(out / "fi001.dat"))
+    ref = cases.golden_vec("bubble_2D_fi001.dat.xz")
+    assert np.array_equal(fi, ref), "max |diff| %g" % np.abs(fi - ref).max()
+    # the initial state output (step 0) holds the unforced-equilibrium masses
+    fi0 = txg.read_petsc_vec(str(out / "fi000.dat")).reshape(-1, 9, 2)
+    assert abs(fi0.sum(axis=(0, 1))[0] - (53 * 53 * 0.03 + (16384 - 53 * 53) * 0.97)) < 1e-9
+    rho = txg.read_petsc_vec(str(out / "rho001.dat")).reshape(-1, 2)
+    assert np.allclose(rho, ref.reshape(-1, 9, 2).sum(axis=1), rtol=1e-14, atol=0)
