@@ -164,6 +164,16 @@ int txg_delta_norm(txg_handle* h, double* norm);
 int txg_profile_enable(txg_handle* h, int32_t on);
 int txg_profile_read(txg_handle* h, double* ms, int32_t* n);
 
+/* Step kernel selection (no counterpart in the reference; for tests, the
+ * benchmark and the tile autotuner).  kernel: 0 automatic (one-pass where
+ * available), 1 two-pass (density pass + collide pass), 2 one-pass
+ * (D3Q19: z-marching tiles with the stencil density in LDS).  tile_x x
+ * tile_y must be an instantiated tile shape (32x8 or 64x4) or 0 x 0 for
+ * the default; zseg is the z-planes per workgroup (0: default).  Returns
+ * TXG_ERR_UNSUPPORTED for a kernel or tile that does not exist for the
+ * handle's lattice. */
+int txg_tune(txg_handle* h, int32_t kernel, int32_t tile_x, int32_t tile_y, int32_t zseg);
+
 /* Wait for all queued work on the handle's stream. */
 int txg_sync(txg_handle* h);
 

@@ -145,6 +145,10 @@ struct txg_handle {
   std::vector<std::pair<int, int>> ev_pairs[2];  // indices into ev_pool per pass
   int walls_set = 0, state_set = 0, fi_set = 0, initialized = 0;
   int local_state = 0;  // f[cur] holds post-stream populations (after init)
+  int rho_valid = 0;    // rho[] holds the stencil density of the current state
+  int kernel = 0;       // txg_tune: 0 auto, 1 two-pass, 2 one-pass
+  int tile = 0;         // index into kTileX/kTileY
+  int zseg = 0;         // z-planes per one-pass workgroup (0: default)
   std::string err;
 
   int fail(int code, const char* fmt, ...) {
@@ -229,6 +233,10 @@ void fill_params(const txg_desc& d, int N, int Np, int flags, int dirichlet, Par
     p.two_cs2[m] = (R)(2. * cs2);
     p.two_cs2_sq[m] = (R)(2. * cs2 * cs2);
     p.feq0a[m] = (R)((1. + dk * 5.) / 6.);
+    p.omega[m] = (R)(1.0 / tau);
+    p.ic2[m] = (R)(1.0 / cs2);
+    p.ic45[m] = (R)(1.0 / (2. * cs2 * cs2));
+    p.ic15[m] = (R)(1.0 / (2. * cs2));
     const double rates[7] = {d.s_c[m], d.s_e[m], d.s_e2[m], d.s_q[m], d.s_nu[m], d.s_pi[m], d.s_m[m]};
     for (int k = 0; k < 7; k++) p.rate[m][k] = (R)rates[k];
     for (int n = 0; n < C; n++) p.g[m][n] = (R)d.g[m][n];
@@ -269,6 +277,42 @@ int ensure_export(txg_handle* h) {
   return TXG_OK;
 }
 
+bool use_fused(const txg_handle* h) {
+  if (h->kernel == 1) return false;
+  return h->ks.fused[h->tile] != nullptr;
+}
+
+FusedGeom fused_geom(const txg_handle* h) {
+  FusedGeom g;
+  const int tx = kTileX[h->tile], ty = kTileY[h->tile];
+  const int nz = h->D == 3 ? h->d.nz : 1;
+  g.ntx = (h->d.nx + tx - 1) / tx;
+  g.nty = (h->d.ny + ty - 1) / ty;
+  g.ntiles = g.ntx * g.nty;
+  int zs = h->zseg;
+  if (zs <= 0) {
+    // aim for about 4 workgroups per CU on 256 CUs, segments of >= 8 planes
+    const int want = 1024;
+    int nseg = std::max(1, want / std::max(1, g.ntiles));
+    zs = std::max(8, (nz + nseg - 1) / nseg);
+  }
+  g.zseg = std::min(zs, nz);
+  g.nseg = (nz + g.zseg - 1) / g.zseg;
+  g.nblocks = g.ntiles * g.nseg;
+  return g;
+}
+
+// make rho[] the stencil density of the current state (the one-pass step
+// keeps it in LDS only)
+int ensure_rho(txg_handle* h) {
+  if (h->local_state || h->rho_valid) return TXG_OK;
+  AnyBufs b = bufs(h, h->cur);
+  int rc = h->hip(h->ks.density(h->params(), b.ptr, 1, 1, h->N, h->stream), "density kernel");
+  if (rc) return rc;
+  h->rho_valid = 1;
+  return TXG_OK;
+}
+
 ExportPtrs export_ptrs(txg_handle* h) {
   ExportPtrs e;
   size_t N = h->N;
@@ -285,6 +329,8 @@ ExportPtrs export_ptrs(txg_handle* h) {
 int run_export(txg_handle* h) {
   if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
   int rc = ensure_export(h);
+  if (rc) return rc;
+  rc = ensure_rho(h);
   if (rc) return rc;
   ExportPtrs e = export_ptrs(h);
   AnyBufs b = bufs(h, h->cur);
@@ -612,6 +658,7 @@ int txg_init(txg_handle* h) {
   rc = h->hip(h->ks.density(h->params(), b.ptr, 0, 0, h->N, h->stream), "density kernel");
   if (rc) return rc;
   h->local_state = 1;
+  h->rho_valid = 1;
   h->initialized = 1;
   h->have_old = 0;
   return h->hip(hipStreamSynchronize(h->stream), "init sync");
@@ -648,10 +695,26 @@ struct ProfScope {
 int txg_step(txg_handle* h, int32_t nsteps) {
   if (!h || nsteps < 0) return TXG_ERR_ARG;
   if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
+  const bool fused = use_fused(h);
+  const FusedGeom g = fused ? fused_geom(h) : FusedGeom{};
   for (int s = 0; s < nsteps; s++) {
+    int rc;
+    if (fused && !h->local_state) {
+      // one pass: pull, density (LDS), forces, collide
+      AnyBufs b = bufs(h, h->cur);
+      {
+        ProfScope ps(h, 0);
+        rc = h->hip(h->ks.fused[h->tile](h->params(), b.ptr, &g, h->stream), "one-pass kernel");
+      }
+      if (rc) return rc;
+      h->cur = 1 - h->cur;
+      h->rho_valid = 0;
+      continue;
+    }
+    // two passes: collide (from the local initial state, or pulled), then
+    // the density of the new state for the next collide
     AnyBufs b = bufs(h, h->cur);
     int pulled = h->local_state ? 0 : 1;
-    int rc;
     {
       ProfScope ps(h, 0);
       rc = h->hip(h->ks.collide(h->params(), b.ptr, pulled, pulled, nullptr, h->N, h->stream), "collide kernel");
@@ -659,13 +722,35 @@ int txg_step(txg_handle* h, int32_t nsteps) {
     if (rc) return rc;
     h->cur = 1 - h->cur;
     h->local_state = 0;
+    h->rho_valid = 0;
+    if (fused) continue;
     AnyBufs b2 = bufs(h, h->cur);
     {
       ProfScope ps(h, 1);
       rc = h->hip(h->ks.density(h->params(), b2.ptr, 1, 1, h->N, h->stream), "density kernel");
     }
     if (rc) return rc;
+    h->rho_valid = 1;
   }
+  return TXG_OK;
+}
+
+int txg_tune(txg_handle* h, int32_t kernel, int32_t tile_x, int32_t tile_y, int32_t zseg) {
+  if (!h || kernel < 0 || kernel > 2 || zseg < 0) return TXG_ERR_ARG;
+  int tile = h->tile;
+  if (tile_x != 0 || tile_y != 0) {
+    tile = -1;
+    for (int i = 0; i < kNumTiles; i++)
+      if (kTileX[i] == tile_x && kTileY[i] == tile_y) tile = i;
+    if (tile < 0) return h->fail(TXG_ERR_UNSUPPORTED, "tile %dx%d is not instantiated", tile_x, tile_y);
+  }
+  if (kernel == 2 && !h->ks.fused[tile])
+    return h->fail(TXG_ERR_UNSUPPORTED, "no one-pass kernel for this lattice");
+  int rc = h->hip(hipStreamSynchronize(h->stream), "tune sync");
+  if (rc) return rc;
+  h->kernel = kernel;
+  h->tile = tile;
+  h->zseg = zseg;
   return TXG_OK;
 }
 

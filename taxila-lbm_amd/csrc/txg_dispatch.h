@@ -14,12 +14,21 @@ struct ExportPtrs {
   double* j;
 };
 
+struct FusedGeom;
+
+// tile shapes of the one-pass D3Q19 kernel (index -> TX x TY)
+constexpr int kNumTiles = 2;
+constexpr int kTileX[kNumTiles] = {32, 64};
+constexpr int kTileY[kNumTiles] = {8, 4};
+
 struct KernelSet {
   // p: Params<R>*, b: Bufs<S,R>*
   hipError_t (*density)(const void* p, const void* b, int pulled, int apply_bc, int N, hipStream_t s);
   hipError_t (*collide)(const void* p, const void* b, int pulled, int apply_bc, const ExportPtrs* ex, int N,
                         hipStream_t s);
   hipError_t (*fiinit)(const void* p, const void* b, const void* u_ic, int N, hipStream_t s);
+  // one-pass step (D3Q19 only; null otherwise), per tile shape
+  hipError_t (*fused[kNumTiles])(const void* p, const void* b, const FusedGeom* g, hipStream_t s);
 };
 
 // returns false if the configuration is not instantiated

@@ -75,6 +75,11 @@ struct Params {
   R kax, kdg;          // fluid-solid 1./6. (1./3.) and 1./12. as default reals
   R bcrho[6][MAXC];
   R shift0[MAXC];      // FP16 storage: stored = f - w_n * shift0[m]
+  // reciprocals used by the FAST (FP32) arithmetic
+  R omega[MAXC];       // 1/tau
+  R ic2[MAXC];         // 1/c_s2
+  R ic45[MAXC];        // 1/(2 c_s2^2)
+  R ic15[MAXC];        // 1/(2 c_s2)
 };
 
 template <class S, class R>
@@ -112,46 +117,58 @@ __device__ __forceinline__ void sfor(F&& f) {
 }
 
 // ------------------------------------------------------------ geometry
-// Wrapped neighbour rows of a node: xw[3+k] = linear offset contribution of
+// Wrapped neighbour rows of a node: xw[H+k] = linear offset contribution of
 // x+k (or -1 when x+k leaves a non-periodic domain), likewise y and z.
-struct Nbr {
-  int xw[7], yw[7], zw[7];
+template <int H>
+struct NbrH {
+  int xw[2 * H + 1], yw[2 * H + 1], zw[2 * H + 1];
 };
+using Nbr = NbrH<3>;
 
-template <int D, class R>
-__device__ __forceinline__ void make_nbr(const Params<R>& p, int x, int y, int z, Nbr& nb) {
+__device__ __forceinline__ int wrap_coord(int v, int n, int periodic) {
+  if (v < 0) return periodic ? v + n : -1;
+  if (v >= n) return periodic ? v - n : -1;
+  return v;
+}
+
+template <int D, int H, class R>
+__device__ __forceinline__ void set_z(const Params<R>& p, int z, NbrH<H>& nb) {
 #pragma unroll
-  for (int k = -3; k <= 3; k++) {
-    int xx = x + k, yy = y + k, zz = z + k;
-    if (xx < 0) xx = p.periodic[0] ? xx + p.nx : -1;
-    else if (xx >= p.nx) xx = p.periodic[0] ? xx - p.nx : -1;
-    if (yy < 0) yy = p.periodic[1] ? yy + p.ny : -1;
-    else if (yy >= p.ny) yy = p.periodic[1] ? yy - p.ny : -1;
-    nb.xw[k + 3] = xx;
-    nb.yw[k + 3] = yy < 0 ? -1 : yy * p.nx;
+  for (int k = -H; k <= H; k++) {
     if constexpr (D == 3) {
-      if (zz < 0) zz = p.periodic[2] ? zz + p.nz : -1;
-      else if (zz >= p.nz) zz = p.periodic[2] ? zz - p.nz : -1;
-      nb.zw[k + 3] = zz < 0 ? -1 : zz * p.nxy;
+      int zz = wrap_coord(z + k, p.nz, p.periodic[2]);
+      nb.zw[k + H] = zz < 0 ? -1 : zz * p.nxy;
     } else {
-      nb.zw[k + 3] = (k == 0) ? 0 : -1;
+      nb.zw[k + H] = (k == 0) ? 0 : -1;
     }
   }
 }
 
+template <int D, int H, class R>
+__device__ __forceinline__ void make_nbr(const Params<R>& p, int x, int y, int z, NbrH<H>& nb) {
+#pragma unroll
+  for (int k = -H; k <= H; k++) {
+    nb.xw[k + H] = wrap_coord(x + k, p.nx, p.periodic[0]);
+    int yy = wrap_coord(y + k, p.ny, p.periodic[1]);
+    nb.yw[k + H] = yy < 0 ? -1 : yy * p.nx;
+  }
+  set_z<D, H, R>(p, z, nb);
+}
+
 // neighbour index at offset (ex,ey,ez), or -1 outside a non-periodic domain
-template <int EX, int EY, int EZ>
-__device__ __forceinline__ int nidx(const Nbr& nb) {
-  int a = nb.xw[EX + 3], b = nb.yw[EY + 3], c = nb.zw[EZ + 3];
+template <int EX, int EY, int EZ, int H>
+__device__ __forceinline__ int nidx(const NbrH<H>& nb) {
+  static_assert(EX >= -H && EX <= H && EY >= -H && EY <= H && EZ >= -H && EZ <= H, "offset beyond rows");
+  int a = nb.xw[EX + H], b = nb.yw[EY + H], c = nb.zw[EZ + H];
   return (a < 0 || b < 0 || c < 0) ? -1 : a + b + c;
 }
 
 // ------------------------------------------------------------ pull-stream
 // f_n(x) = f*_n(x - c_n), or f*_opp(n)(x) when x - c_n is a wall
 // (DistributionStream + DistributionBounceback, lbm_distribution_function.F90:560-784)
-template <int D, int C, class S, class R>
+template <int D, int C, class S, class R, int H>
 __device__ __forceinline__ void pull(const Params<R>& p, const S* __restrict__ fin, int node,
-                                     uint32_t mk, const Nbr& nb, R (&f)[C][Lat<D>::Q]) {
+                                     uint32_t mk, const NbrH<H>& nb, R (&f)[C][Lat<D>::Q]) {
   using L = Lat<D>;
   sfor<0, L::Q>([&](auto NI) {
     constexpr int n = decltype(NI)::value;
@@ -161,7 +178,7 @@ __device__ __forceinline__ void pull(const Params<R>& p, const S* __restrict__ f
       src = node;
       dir = L::opp[n];
     } else {
-      src = nidx<-L::cx[n], -L::cy[n], -L::cz[n]>(nb);
+      src = nidx<-L::cx[n], -L::cy[n], -L::cz[n], H>(nb);
       dir = n;
     }
     // w_opp(n) = w_n, so the storage shift does not depend on the branch
@@ -335,16 +352,30 @@ __device__ __forceinline__ bool is_num(R v) {
   return v == v;
 }
 
-template <class R>
-__device__ __forceinline__ bool fluid_at(const R* __restrict__ rho0, int idx) {
-  return idx >= 0 && is_num(rho0[idx]);
-}
+// Stencil density in global memory (two-pass step): rho[m*Np + node],
+// NaN on walls, neighbours addressed through the node's rows.
+template <class R, int H>
+struct GlobalRho {
+  const R* __restrict__ rho;
+  int Np;
+  const NbrH<H>& nb;
+  template <int X, int Y, int Z>
+  __device__ __forceinline__ bool fluid() const {
+    const int i = nidx<X, Y, Z, H>(nb);
+    return i >= 0 && is_num(rho[i]);
+  }
+  template <int X, int Y, int Z>
+  __device__ __forceinline__ R get(int m) const {
+    return rho[(size_t)m * Np + nidx<X, Y, Z, H>(nb)];
+  }
+};
 
 // psi-gradient (fluid-fluid) force, LBMAddFluidFluidForcesD3/D2
-// (lbm_forcing.F90:51-1299), psi = rho
-template <int D, int C, int NFF, class R>
-__device__ __forceinline__ void ff_force(const Params<R>& p, const R* __restrict__ rho, int node,
-                                         const Nbr& nb, bool clean, const R (&rs)[C], R (&F)[C][3]) {
+// (lbm_forcing.F90:51-1299), psi = rho.  A provides fluid<X,Y,Z>() and
+// get<X,Y,Z>(m) of the stencil density.
+template <int D, int C, int NFF, bool FAST, class R, class A>
+__device__ __forceinline__ void ff_force(const Params<R>& p, const A& acc, bool clean, const R (&rs)[C],
+                                         R (&F)[C][3]) {
   R grad[C][3];
   R ws[3] = {(R)0, (R)0, (R)0};
 #pragma unroll
@@ -352,19 +383,18 @@ __device__ __forceinline__ void ff_force(const Params<R>& p, const R* __restrict
   sfor<0, NFF>([&](auto II) {
     constexpr int I = decltype(II)::value;
     using E = FFOff<D, I>;
-    const int t = nidx<E::x, E::y, E::z>(nb);
     bool ok;
     if (clean) {
       ok = true;
     } else {
-      ok = fluid_at(rho, t);
+      ok = acc.template fluid<E::x, E::y, E::z>();
       if constexpr (E::cond.nalt > 0) {
         bool any = false;
         sfor<0, E::cond.nalt>([&](auto AI) {
           constexpr int a = decltype(AI)::value;
-          bool all = fluid_at(rho, nidx<E::cond.c[a][0][0], E::cond.c[a][0][1], E::cond.c[a][0][2]>(nb));
+          bool all = acc.template fluid<E::cond.c[a][0][0], E::cond.c[a][0][1], E::cond.c[a][0][2]>();
           if constexpr (E::cond.len[a] > 1)
-            all = all && fluid_at(rho, nidx<E::cond.c[a][1][0], E::cond.c[a][1][1], E::cond.c[a][1][2]>(nb));
+            all = all && acc.template fluid<E::cond.c[a][1][0], E::cond.c[a][1][1], E::cond.c[a][1][2]>();
           any = any || all;
         });
         ok = ok && any;
@@ -373,7 +403,7 @@ __device__ __forceinline__ void ff_force(const Params<R>& p, const R* __restrict
     if (ok) {
       R dr[C];
 #pragma unroll
-      for (int m = 0; m < C; m++) dr[m] = rho[(size_t)m * p.Np + t] - rs[m];
+      for (int m = 0; m < C; m++) dr[m] = acc.template get<E::x, E::y, E::z>(m) - rs[m];
       sfor<0, D>([&](auto DI) {
         constexpr int d = decltype(DI)::value;
         constexpr int c = d == 0 ? E::x : (d == 1 ? E::y : E::z);
@@ -393,26 +423,38 @@ __device__ __forceinline__ void ff_force(const Params<R>& p, const R* __restrict
     }
   });
 #pragma unroll
-  for (int m = 0; m < C; m++)
+  for (int d = 0; d < D; d++)
+    if (ws[d] > p.ffeps) {
+      if constexpr (FAST) {
+        const R iw = (R)1 / ws[d];
 #pragma unroll
-    for (int d = 0; d < D; d++)
-      if (ws[d] > p.ffeps) {
-        R s = (R)0;
+        for (int m = 0; m < C; m++) {
+          R s = (R)0;
 #pragma unroll
-        for (int mp = 0; mp < C; mp++) s = s + p.g[m][mp] * (grad[mp][d] / ws[d]);
-        F[m][d] = F[m][d] - p.c0 * rs[m] * s;
+          for (int mp = 0; mp < C; mp++) s = s + p.g[m][mp] * grad[mp][d];
+          F[m][d] = F[m][d] - p.c0 * rs[m] * s * iw;
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < C; m++) {
+          R s = (R)0;
+#pragma unroll
+          for (int mp = 0; mp < C; mp++) s = s + p.g[m][mp] * (grad[mp][d] / ws[d]);
+          F[m][d] = F[m][d] - p.c0 * rs[m] * s;
+        }
       }
+    }
 }
 
 // fluid-solid (wettability) force, LBMAddFluidSolidForcesD3/D2
 // (lbm_forcing.F90:1326-1421): for each neighbour x + c_n that is a mineral
-template <int D, int C, class R>
-__device__ __forceinline__ void fs_force(const Params<R>& p, const uint8_t* __restrict__ tag, const Nbr& nb,
+template <int D, int C, class R, int H>
+__device__ __forceinline__ void fs_force(const Params<R>& p, const uint8_t* __restrict__ tag, const NbrH<H>& nb,
                                          const R (&rs)[C], R (&F)[C][3]) {
   using L = Lat<D>;
   sfor<1, L::Q>([&](auto NI) {
     constexpr int n = decltype(NI)::value;
-    const int t = nidx<L::cx[n], L::cy[n], L::cz[n]>(nb);
+    const int t = nidx<L::cx[n], L::cy[n], L::cz[n], H>(nb);
     const int mineral = t >= 0 ? (int)tag[t] : 0;
     if (mineral > 0) {
       const R k = n <= L::NAX ? p.kax : p.kdg;
@@ -483,10 +525,10 @@ __global__ void __launch_bounds__(256) k_density(Params<R> p, Bufs<S, R> b, int 
 }
 
 // Moments, forces, common velocity and collision of one node.
-// mode 0: step (pull from fin, Dirichlet BCs, collide into fout)
-// mode 1: initial collision (local read, no BCs)
-// mode 2: state export (pull + BCs unless local, no collision; writes f, rho,
-//         u', F, j in FP64 to the export arrays)
+// k_collide modes: pulled=1, apply_bc=1 for a step (pull from fin, Dirichlet
+// BCs, collide into fout); pulled=0 for the first collision after init
+// (local read, no BCs); ex.f != nullptr exports the state (f, rho, u', F, j
+// in FP64) instead of colliding.
 template <int D, int C, class R>
 struct Export {
   double* f;     // [(m*Q+n)*N + node]
@@ -496,7 +538,189 @@ struct Export {
   double* j;     // [(m*D+d)*N + node]
 };
 
-template <int D, int C, int NFF, class S, class R>
+// rho after BCUpdateRho and j (DistributionCalcDensity/CalcFlux), forces
+// from the stencil density (FlowCalcForces, lbm_flow.F90:760-808) and the
+// common velocity (FlowUpdateUE, lbm_flow.F90:494-573)
+template <int D, int C, int NFF, bool FAST, class R, class A, int H>
+__device__ __forceinline__ void moments_forces(const Params<R>& p, const uint8_t* __restrict__ tag, uint32_t mk,
+                                               const NbrH<H>& nb, const R (&f)[C][Lat<D>::Q], const R (&rs)[C],
+                                               const A& acc, R (&rl)[C], R (&j)[C][3], R (&F)[C][3], R (&up)[3]) {
+  constexpr int Q = Lat<D>::Q;
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R s = (R)0;
+#pragma unroll
+    for (int n = 0; n < Q; n++) s = s + f[m][n];
+    rl[m] = s;
+    sfor<0, D>([&](auto DI) {
+      constexpr int d = decltype(DI)::value;
+      R a = (R)0;
+      sfor<0, Q>([&](auto NI) {
+        constexpr int n = decltype(NI)::value;
+        constexpr int c = cdir<D>(n, d);
+        if constexpr (c == 1) a = a + f[m][n];
+        if constexpr (c == -1) a = a - f[m][n];
+      });
+      j[m][d] = a;
+    });
+  }
+#pragma unroll
+  for (int m = 0; m < C; m++) F[m][0] = F[m][1] = F[m][2] = (R)0;
+  if ((p.flags & kFlagFS) && (mk & kFSNear)) fs_force<D, C, R, H>(p, tag, nb, rs, F);
+  if (p.flags & kFlagBody) {
+#pragma unroll
+    for (int m = 0; m < C; m++)
+#pragma unroll
+      for (int d = 0; d < D; d++) F[m][d] = F[m][d] + p.gvt[d] * p.mm[m] * rs[m];
+  }
+  if constexpr (NFF > 0) {
+    if (p.flags & kFlagFF) ff_force<D, C, NFF, FAST, R>(p, acc, (mk & kFFClean) != 0, rs, F);
+  }
+  R den = (R)0;
+#pragma unroll
+  for (int m = 0; m < C; m++) den = den + rl[m] * p.mmot[m];
+  const R iden = (R)1 / den;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    R num = (R)0;
+#pragma unroll
+    for (int m = 0; m < C; m++) num = num + (j[m][d] + (R)0.5 * F[m][d]) * p.mmot[m];
+    if constexpr (FAST)
+      up[d] = num * iden;
+    else
+      up[d] = num / den;
+  }
+}
+
+// collision and store of f*(node) (FlowCollisionD3/D2 lbm_flow.F90:960-1029,
+// equilibrium DiscretizationEquilf lbm_discretization_d3q19.F90:265-305 /
+// _d2q9.F90:168-205, RelaxationCollideSRT/MRT lbm_relaxation.F90:171-200).
+// EXACT (FAST = false) keeps the reference's operations and their order;
+// FAST folds the divisions into reciprocals and the SRT update into
+// f' = (1-w) f + feq (w + pref (1 - w/2)).
+template <int D, int C, bool FAST, class S, class R>
+__device__ __forceinline__ void collide_store(const Params<R>& p, S* __restrict__ fout, int node,
+                                              R (&f)[C][Lat<D>::Q], const R (&rl)[C], const R (&F)[C][3],
+                                              const R (&up)[3]) {
+  using L = Lat<D>;
+  constexpr int Q = L::Q;
+  R usqr = (R)0;
+#pragma unroll
+  for (int d = 0; d < D; d++) usqr = usqr + up[d] * up[d];
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    const R r = rl[m];
+    const R rcs = r * p.cs2[m];
+    R feq[Q], pref[Q];
+    if constexpr (FAST) {
+      const R irc = (R)1 / rcs;
+      R Fu = (R)0;
+#pragma unroll
+      for (int d = 0; d < D; d++) Fu = Fu + F[m][d] * up[d];
+      const R A0 = p.a15[m] - usqr * p.ic15[m];
+      sfor<0, Q>([&](auto NI) {
+        constexpr int n = decltype(NI)::value;
+        R Fc = (R)0, ue = (R)0;
+        sfor<0, D>([&](auto DI) {
+          constexpr int d = decltype(DI)::value;
+          constexpr int c = cdir<D>(n, d);
+          if constexpr (c == 1) {
+            Fc = Fc + F[m][d];
+            ue = ue + up[d];
+          }
+          if constexpr (c == -1) {
+            Fc = Fc - F[m][d];
+            ue = ue - up[d];
+          }
+        });
+        pref[n] = (Fc - Fu) * irc;
+        if constexpr (n == 0) {
+          if constexpr (D == 3)
+            feq[0] = r * (p.dk[m] - usqr * (R)0.5);
+          else
+            feq[0] = r * (p.feq0a[m] - usqr * (R)(2. / 3.));
+        } else {
+          feq[n] = ((R)L::w[n] * r) * (A0 + ue * (p.ic2[m] + ue * p.ic45[m]));
+        }
+      });
+    } else {
+      sfor<0, Q>([&](auto NI) {
+        constexpr int n = decltype(NI)::value;
+        // prefactor: sum_d F_d (c_nd - u_d) / (rho c_s^2)
+        R s = (R)0;
+        sfor<0, D>([&](auto DI) {
+          constexpr int d = decltype(DI)::value;
+          s = s + F[m][d] * ((R)cdir<D>(n, d) - up[d]);
+        });
+        pref[n] = s / rcs;
+        if constexpr (n == 0) {
+          if constexpr (D == 3)
+            feq[0] = r * (p.dk[m] - usqr / (R)2.);
+          else
+            feq[0] = r * (p.feq0a[m] - (R)2. * usqr / (R)3.);
+        } else {
+          R ue = (R)0;
+          sfor<0, D>([&](auto DI) {
+            constexpr int d = decltype(DI)::value;
+            constexpr int c = cdir<D>(n, d);
+            if constexpr (c == 1) ue = ue + up[d];
+            if constexpr (c == -1) ue = ue - up[d];
+          });
+          feq[n] = (R)L::w[n] * r *
+                   (p.a15[m] + ue / p.cs2[m] + ue * ue / p.two_cs2_sq[m] - usqr / p.two_cs2[m]);
+        }
+      });
+    }
+    if (!(p.flags & kFlagMRT)) {
+      if constexpr (FAST) {
+        const R w = p.omega[m], w1 = (R)1 - w, wh = (R)1 - (R)0.5 * w;
+#pragma unroll
+        for (int n = 0; n < Q; n++) f[m][n] = w1 * f[m][n] + feq[n] * (w + pref[n] * wh);
+      } else {
+#pragma unroll
+        for (int n = 0; n < Q; n++) {
+          const R bar = ((R)1. - (R)0.5 * pref[n]) * feq[n];
+          f[m][n] = f[m][n] - (f[m][n] - bar) / p.tau[m];
+        }
+#pragma unroll
+        for (int n = 0; n < Q; n++) f[m][n] = f[m][n] + pref[n] * feq[n];
+      }
+    } else {
+      R dfi[Q];
+#pragma unroll
+      for (int n = 0; n < Q; n++) dfi[n] = f[m][n] - ((R)1. - (R)0.5 * pref[n]) * feq[n];
+      sfor<0, Q>([&](auto KI) {
+        constexpr int k = decltype(KI)::value;
+        R md = (R)0;
+        sfor<0, Q>([&](auto NI) {
+          constexpr int n = decltype(NI)::value;
+          constexpr int mv = L::M[k][n];
+          if constexpr (mv == 1) md = md + dfi[n];
+          else if constexpr (mv == -1) md = md - dfi[n];
+          else if constexpr (mv != 0) md = md + (R)mv * dfi[n];
+        });
+        R t;
+        if constexpr (FAST)
+          t = p.rate[m][L::rate[k]] * md * (R)(1.0 / L::mmt[k]);
+        else
+          t = p.rate[m][L::rate[k]] * md / (R)L::mmt[k];
+        sfor<0, Q>([&](auto NI) {
+          constexpr int n = decltype(NI)::value;
+          constexpr int mv = L::M[k][n];
+          if constexpr (mv == 1) f[m][n] = f[m][n] - t;
+          else if constexpr (mv == -1) f[m][n] = f[m][n] + t;
+          else if constexpr (mv != 0) f[m][n] = f[m][n] - t * (R)mv;
+        });
+      });
+#pragma unroll
+      for (int n = 0; n < Q; n++) f[m][n] = f[m][n] + pref[n] * feq[n];
+    }
+#pragma unroll
+    for (int n = 0; n < Q; n++) st<S, R>(fout + (size_t)(m * Q + n) * p.Np, node, f[m][n], (R)L::w[n] * p.shift0[m]);
+  }
+}
+
+template <int D, int C, int NFF, bool FAST, class S, class R>
 __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int pulled, int apply_bc,
                                                  Export<D, C, R> ex) {
   using L = Lat<D>;
@@ -532,55 +756,12 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
   else
     load_local<D, C, S, R>(p, b.fin, node, f);
   if (apply_bc && (mk & (0x3fu << kFaceShift))) apply_dirichlet<D, C, R>(p, mk, f);
-
-  // rho after BCUpdateRho, and j (DistributionCalcDensity/CalcFlux)
-  R rl[C], j[C][3];
-#pragma unroll
-  for (int m = 0; m < C; m++) {
-    R s = (R)0;
-#pragma unroll
-    for (int n = 0; n < Q; n++) s = s + f[m][n];
-    rl[m] = s;
-    sfor<0, D>([&](auto DI) {
-      constexpr int d = decltype(DI)::value;
-      R a = (R)0;
-      sfor<0, Q>([&](auto NI) {
-        constexpr int n = decltype(NI)::value;
-        constexpr int c = cdir<D>(n, d);
-        if constexpr (c == 1) a = a + f[m][n];
-        if constexpr (c == -1) a = a - f[m][n];
-      });
-      j[m][d] = a;
-    });
-  }
-  // forces from the stencil rho (FlowCalcForces, lbm_flow.F90:760-808)
   R rs[C];
 #pragma unroll
   for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
-  R F[C][3];
-#pragma unroll
-  for (int m = 0; m < C; m++) F[m][0] = F[m][1] = F[m][2] = (R)0;
-  if ((p.flags & kFlagFS) && (mk & kFSNear)) fs_force<D, C, R>(p, b.tag, nb, rs, F);
-  if (p.flags & kFlagBody) {
-#pragma unroll
-    for (int m = 0; m < C; m++)
-#pragma unroll
-      for (int d = 0; d < D; d++) F[m][d] = F[m][d] + p.gvt[d] * p.mm[m] * rs[m];
-  }
-  if constexpr (NFF > 0) {
-    if (p.flags & kFlagFF) ff_force<D, C, NFF, R>(p, b.rho, node, nb, (mk & kFFClean) != 0, rs, F);
-  }
-  // common velocity (FlowUpdateUE, lbm_flow.F90:494-573)
-  R up[3];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    R num = (R)0, den = (R)0;
-#pragma unroll
-    for (int m = 0; m < C; m++) num = num + (j[m][d] + (R)0.5 * F[m][d]) * p.mmot[m];
-#pragma unroll
-    for (int m = 0; m < C; m++) den = den + rl[m] * p.mmot[m];
-    up[d] = num / den;
-  }
+  GlobalRho<R, 3> acc{b.rho, p.Np, nb};
+  R rl[C], j[C][3], F[C][3], up[3];
+  moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
   if (ex.f) {  // state export
 #pragma unroll
     for (int m = 0; m < C; m++) {
@@ -597,79 +778,7 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
     for (int d = 0; d < D; d++) ex.up[(size_t)d * p.N + node] = (double)up[d];
     return;
   }
-  // collision (FlowCollisionD3/D2 lbm_flow.F90:960-1029, equilibrium
-  // DiscretizationEquilf lbm_discretization_d3q19.F90:265-305 / _d2q9.F90:168-205,
-  // RelaxationCollideSRT/MRT lbm_relaxation.F90:171-200)
-  R usqr = (R)0;
-#pragma unroll
-  for (int d = 0; d < D; d++) usqr = usqr + up[d] * up[d];
-#pragma unroll
-  for (int m = 0; m < C; m++) {
-    const R r = rl[m];
-    const R rcs = r * p.cs2[m];
-    R feq[Q], pref[Q];
-    sfor<0, Q>([&](auto NI) {
-      constexpr int n = decltype(NI)::value;
-      // prefactor: sum_d F_d (c_nd - u_d) / (rho c_s^2)
-      R s = (R)0;
-      sfor<0, D>([&](auto DI) {
-        constexpr int d = decltype(DI)::value;
-        s = s + F[m][d] * ((R)cdir<D>(n, d) - up[d]);
-      });
-      pref[n] = s / rcs;
-      if constexpr (n == 0) {
-        if constexpr (D == 3)
-          feq[0] = r * (p.dk[m] - usqr / (R)2.);
-        else
-          feq[0] = r * (p.feq0a[m] - (R)2. * usqr / (R)3.);
-      } else {
-        R ue = (R)0;
-        sfor<0, D>([&](auto DI) {
-          constexpr int d = decltype(DI)::value;
-          constexpr int c = cdir<D>(n, d);
-          if constexpr (c == 1) ue = ue + up[d];
-          if constexpr (c == -1) ue = ue - up[d];
-        });
-        feq[n] = (R)L::w[n] * r *
-                 (p.a15[m] + ue / p.cs2[m] + ue * ue / p.two_cs2_sq[m] - usqr / p.two_cs2[m]);
-      }
-    });
-    if (!(p.flags & kFlagMRT)) {
-#pragma unroll
-      for (int n = 0; n < Q; n++) {
-        const R bar = ((R)1. - (R)0.5 * pref[n]) * feq[n];
-        f[m][n] = f[m][n] - (f[m][n] - bar) / p.tau[m];
-      }
-    } else {
-      R dfi[Q];
-#pragma unroll
-      for (int n = 0; n < Q; n++) dfi[n] = f[m][n] - ((R)1. - (R)0.5 * pref[n]) * feq[n];
-      sfor<0, Q>([&](auto KI) {
-        constexpr int k = decltype(KI)::value;
-        R md = (R)0;
-        sfor<0, Q>([&](auto NI) {
-          constexpr int n = decltype(NI)::value;
-          constexpr int mv = L::M[k][n];
-          if constexpr (mv == 1) md = md + dfi[n];
-          else if constexpr (mv == -1) md = md - dfi[n];
-          else if constexpr (mv != 0) md = md + (R)mv * dfi[n];
-        });
-        const R t = p.rate[m][L::rate[k]] * md / (R)L::mmt[k];
-        sfor<0, Q>([&](auto NI) {
-          constexpr int n = decltype(NI)::value;
-          constexpr int mv = L::M[k][n];
-          if constexpr (mv == 1) f[m][n] = f[m][n] - t;
-          else if constexpr (mv == -1) f[m][n] = f[m][n] + t;
-          else if constexpr (mv != 0) f[m][n] = f[m][n] - t * (R)mv;
-        });
-      });
-    }
-#pragma unroll
-    for (int n = 0; n < Q; n++) {
-      f[m][n] = f[m][n] + pref[n] * feq[n];
-      st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, f[m][n], (R)L::w[n] * p.shift0[m]);
-    }
-  }
+  collide_store<D, C, FAST, S, R>(p, b.fout, node, f, rl, F, up);
 }
 
 // FlowFiInit (lbm_flow.F90:923-934): f = (1 - pref/2) feq(rho_IC, u_IC) with
@@ -706,7 +815,8 @@ __global__ void __launch_bounds__(256) k_fiinit(Params<R> p, Bufs<S, R> b, const
       for (int d = 0; d < D; d++) F[m][d] = F[m][d] + p.gvt[d] * p.mm[m] * rs[m];
   }
   if constexpr (NFF > 0) {
-    if (p.flags & kFlagFF) ff_force<D, C, NFF, R>(p, b.rho, node, nb, (mk & kFFClean) != 0, rs, F);
+    GlobalRho<R, 3> acc{b.rho, p.Np, nb};
+    if (p.flags & kFlagFF) ff_force<D, C, NFF, false, R>(p, acc, (mk & kFFClean) != 0, rs, F);
   }
 #pragma unroll
   for (int m = 0; m < C; m++) {
@@ -747,3 +857,5 @@ __global__ void __launch_bounds__(256) k_fiinit(Params<R> p, Bufs<S, R> b, const
 }
 
 }  // namespace txg
+
+#include "txg_fused.h"

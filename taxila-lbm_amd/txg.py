@@ -158,7 +158,7 @@ _lib = None
 SYMBOLS = ("txg_desc_init", "txg_create", "txg_set_walls", "txg_set_state", "txg_set_fi", "txg_init",
            "txg_step", "txg_get_fi", "txg_get_moments", "txg_diagnostics", "txg_reduce",
            "txg_delta_norm", "txg_sync", "txg_last_error", "txg_destroy", "txg_profile_enable",
-           "txg_profile_read")
+           "txg_profile_read", "txg_tune")
 
 
 class TxgError(RuntimeError):
@@ -187,6 +187,7 @@ def lib():
         L.txg_last_error.argtypes = [vp, C.c_char_p, C.c_int32]
         L.txg_profile_enable.argtypes = [vp, C.c_int32]
         L.txg_profile_read.argtypes = [vp, vp, vp]
+        L.txg_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         for name in SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -297,3 +298,11 @@ class Domain:
         n = np.zeros(2, dtype=np.int32)
         self._check(lib().txg_profile_read(self.h, _p(ms), _p(n)), "txg_profile_read")
         return {"collide": (float(ms[0]), int(n[0])), "density": (float(ms[1]), int(n[1]))}
+
+    KERNELS = {"auto": 0, "twopass": 1, "onepass": 2}
+
+    def tune(self, kernel="auto", tile=(0, 0), zseg=0):
+        """Select the step kernel ('auto', 'twopass', 'onepass'), the one-pass
+        tile (32, 8) / (64, 4) and the z-planes per workgroup."""
+        self._check(lib().txg_tune(self.h, self.KERNELS[kernel], int(tile[0]), int(tile[1]), int(zseg)),
+                    "txg_tune")

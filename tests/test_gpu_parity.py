@@ -67,10 +67,36 @@ def build_case(name, kw, wmode, icmode):
     return d, walls, rho, u
 
 
+CASES += [
+    ("d3_odd_mixed_bc", dict(ndims=3, ncomponents=2, shape=(37, 23, 19), periodic=(0, 1, 0), isotropy_order=4,
+                             g=[[0, 0.1], [0.1, 0]], tau=[1.0, 0.8], gw=[[0.05, -0.05]],
+                             bc={"xm": [0.98, 0.03], "xp": [0.97, 0.03]}), "porous", "ball", 20),
+    ("d3_odd_iso8_periodic", dict(ndims=3, ncomponents=2, shape=(35, 21, 17), isotropy_order=8,
+                                  g=[[0, 0.1], [0.1, 0]], tau=[1.0, 1.0]), "porous", "ball", 20),
+]
+
+
+def run_gpu(d, walls, rho, u, steps, kernel="auto", tile=(0, 0), zseg=0):
+    dom = txg.Domain(d)
+    dom.tune(kernel, tile, zseg)
+    dom.set_walls(walls)
+    dom.set_state(rho, u)
+    dom.init()
+    if steps:
+        dom.step(steps)
+    return dom
+
+
+KERNELS = ["twopass", "onepass"]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name,kw,wmode,icmode,steps", CASES, ids=[c[0] for c in CASES])
-def test_fp64_bit_exact_vs_oracle(name, kw, wmode, icmode, steps):
+def test_fp64_bit_exact_vs_oracle(name, kw, wmode, icmode, steps, kernel):
     d, walls, rho, u = build_case(name, kw, wmode, icmode)
-    g = cases.run(txg.Domain, d, walls, rho, u, steps)
+    if kernel == "onepass" and d.ndims == 2:
+        pytest.skip("the one-pass kernel is D3Q19 only")
+    g = run_gpu(d, walls, rho, u, steps, kernel)
     o = cases.run(oracle.Oracle, d, walls, rho, u, steps)
     fl = walls == 0
     C, D, Q = d.ncomponents, d.ndims, d.Q
@@ -82,3 +108,29 @@ def test_fp64_bit_exact_vs_oracle(name, kw, wmode, icmode, steps):
     assert np.array_equal(rg[np.repeat(fl, C)], ro[np.repeat(fl, C)])
     assert np.array_equal(ug[np.repeat(fl, C * D)], uo[np.repeat(fl, C * D)])
     assert np.array_equal(Fg[np.repeat(fl, C * D)], Fo[np.repeat(fl, C * D)])
+
+
+@pytest.mark.parametrize("tile,zseg", [((32, 8), 1), ((32, 8), 3), ((64, 4), 0), ((64, 4), 19), ((32, 8), 100)])
+def test_onepass_tiles_and_segments_fp64(tile, zseg):
+    name, kw, wm, icm, steps = CASES[[c[0] for c in CASES].index("d3_odd_mixed_bc")]
+    d, walls, rho, u = build_case(name, kw, wm, icm)
+    a = run_gpu(d, walls, rho, u, 12, "onepass", tile, zseg).get_fi()
+    o = cases.run(oracle.Oracle, d, walls, rho, u, 12).get_fi()
+    m = np.repeat(walls == 0, d.Q * d.ncomponents)
+    assert np.array_equal(a[m], o[m])
+
+
+@pytest.mark.parametrize("prec", [txg.FP32, txg.FP32_FP16])
+def test_fp32_onepass_matches_twopass(prec):
+    """The same FP32 expressions in both kernels; they differ only where the
+    compiler fuses a multiply-add in one kernel and not the other, i.e. at
+    the last-bit level: |diff| <= 1e-6 on populations of order 1e-3..0.4
+    after 25 steps (FP16 storage: one half-precision ulp of the shifted
+    value, 2^-11 * 0.5)."""
+    name, kw, wm, icm, steps = CASES[[c[0] for c in CASES].index("d3_porous_dirichlet")]
+    kw = dict(kw, precision=prec, rho0=[0.5, 0.5])
+    d, walls, rho, u = build_case(name, kw, wm, icm)
+    a = run_gpu(d, walls, rho, u, steps, "twopass").get_fi()
+    b = run_gpu(d, walls, rho, u, steps, "onepass").get_fi()
+    tol = 1e-6 if prec == txg.FP32 else 2.0 ** -11 * 0.5
+    assert np.abs(a - b).max() <= tol
