@@ -279,6 +279,9 @@ int ensure_export(txg_handle* h) {
 
 bool use_fused(const txg_handle* h) {
   if (h->kernel == 1) return false;
+  // the one-pass kernel addresses each component's populations with 32-bit
+  // byte offsets
+  if ((uint64_t)h->Q * h->Np * h->ssize >= (1ull << 32)) return false;
   return h->ks.fused[h->tile] != nullptr;
 }
 
@@ -746,6 +749,8 @@ int txg_tune(txg_handle* h, int32_t kernel, int32_t tile_x, int32_t tile_y, int3
   }
   if (kernel == 2 && !h->ks.fused[tile])
     return h->fail(TXG_ERR_UNSUPPORTED, "no one-pass kernel for this lattice");
+  if (kernel == 2 && (uint64_t)h->Q * h->Np * h->ssize >= (1ull << 32))
+    return h->fail(TXG_ERR_UNSUPPORTED, "one-pass kernel needs < 4 GiB of populations per component");
   int rc = h->hip(hipStreamSynchronize(h->stream), "tune sync");
   if (rc) return rc;
   h->kernel = kernel;

@@ -15,6 +15,12 @@
 // versus read twice for the two-pass step.  Without a fluid-fluid force
 // (r = 0) phase A disappears and the density is the node's own.
 //
+// Populations are addressed through one buffer resource per component
+// (base in SGPRs, 32-bit byte offsets per lane, hardware range check), so a
+// pull costs a few 32-bit adds and a select instead of 64-bit address
+// arithmetic.  This requires Q*Np*sizeof(S) < 4 GiB per component, which
+// the host checks before choosing this kernel.
+//
 // Work decomposition: tiles x z-segments, with the workgroup index remapped
 // so that consecutive tiles of a segment land on the same XCD (the hardware
 // hands workgroups to the 8 XCDs round-robin), keeping ring and re-pull
@@ -22,6 +28,75 @@
 #pragma once
 
 namespace txg {
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+// buffer resource over `bytes` bytes at p (raw addressing, range-checked)
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+template <class S, class R>
+__device__ __forceinline__ R bld(rsrc_t r, uint32_t off, R shift) {
+  if constexpr (std::is_same<S, float>::value) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+  } else if constexpr (std::is_same<S, double>::value) {
+    u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+    return __builtin_bit_cast(double, v);
+  } else {
+    unsigned short u = __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+    return (R)__half2float(__ushort_as_half(u)) + shift;
+  }
+}
+
+template <class S, class R>
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t off, R v, R shift) {
+  if constexpr (std::is_same<S, float>::value) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, 0);
+  } else if constexpr (std::is_same<S, double>::value) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)off, 0, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn((float)(v - shift))), r, (int)off, 0, 0);
+  }
+}
+
+// byte offsets of the +-1 rows of a column: x, y*NX, z*NX*NY times sizeof(S)
+struct RowsB {
+  uint32_t x[3], y[3], z[3];
+};
+
+// pull-stream with bounce-back (DistributionStream + DistributionBounceback,
+// lbm_distribution_function.F90:560-784) through buffer resources
+template <int C, class S, class R>
+__device__ __forceinline__ void pull_buf(const Params<R>& p, const rsrc_t (&rin)[C], uint32_t mk, const RowsB& rb,
+                                         uint32_t plane_b, R (&f)[C][19]) {
+  using L = Lat<3>;
+  const uint32_t node_b = rb.x[1] + rb.y[1] + rb.z[1];
+  sfor<0, 19>([&](auto NI) {
+    constexpr int n = decltype(NI)::value;
+    uint32_t off;
+    if constexpr (n == 0) {
+      off = node_b;
+    } else {
+      const uint32_t so = rb.x[1 - L::cx[n]] + rb.y[1 - L::cy[n]] + rb.z[1 - L::cz[n]] + (uint32_t)n * plane_b;
+      const uint32_t bo = node_b + (uint32_t)L::opp[n] * plane_b;
+      off = (mk & (1u << n)) ? bo : so;
+    }
+#pragma unroll
+    for (int m = 0; m < C; m++) f[m][n] = bld<S, R>(rin[m], off, (R)L::w[n] * p.shift0[m]);
+  });
+}
+
+template <class S, class R>
+struct BufStore {
+  const Params<R>& p;
+  const rsrc_t* rout;
+  uint32_t node_b, plane_b;
+  __device__ __forceinline__ void operator()(int m, int n, R v) const {
+    bst<S, R>(rout[m], node_b + (uint32_t)n * plane_b, v, (R)Lat<3>::w[n] * p.shift0[m]);
+  }
+};
 
 // LDS stencil density: s[(slot*C + m)*EY*EX + row*EX + col]
 template <class R, int C, int RR, int EXW, int EYW>
@@ -60,7 +135,16 @@ __global__ void __launch_bounds__(TX* TY) k_fused3(Params<R> p, Bufs<S, R> b, Fu
   constexpr int EXW = TX + 2 * RR, EYW = TY + 2 * RR, NCOL = EXW * EYW, NPL = 2 * RR + 1;
   constexpr int NT = TX * TY;
   constexpr int NCP = (NCOL + NT - 1) / NT;  // columns per thread in phase A
+  constexpr uint32_t SB = sizeof(S);
   __shared__ R srho[RR > 0 ? NPL * C * NCOL : 1];
+
+  const uint32_t plane_b = (uint32_t)p.Np * SB;
+  rsrc_t rin[C], rout[C];
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    rin[m] = make_rsrc(b.fin + (size_t)m * Q * p.Np, (uint32_t)Q * plane_b);
+    rout[m] = make_rsrc(b.fout + (size_t)m * Q * p.Np, (uint32_t)Q * plane_b);
+  }
 
   const int lb = xcd_remap(blockIdx.x, g.nblocks);
   const int tile = lb % g.ntiles, seg = lb / g.ntiles;
@@ -71,12 +155,20 @@ __global__ void __launch_bounds__(TX* TY) k_fused3(Params<R> p, Bufs<S, R> b, Fu
   const int x = x0 + tx, y = y0 + ty;
   const bool active = x < p.nx && y < p.ny;
 
-  // interior column rows (+-1 for the pull and the wettability force)
+  // rows of the interior column; nb (element units) serves the wettability force
+  RowsB rb;
   NbrH<1> nb;
-  if (active) make_nbr<3, 1, R>(p, x, y, z0, nb);
+  if (active) {
+    make_nbr<3, 1, R>(p, x, y, z0, nb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      rb.x[d] = (uint32_t)nb.xw[d] * SB;
+      rb.y[d] = (uint32_t)nb.yw[d] * SB;
+    }
+  }
 
-  // phase-A columns of this thread: rows of the column and its +-1 neighbours
-  int cxw[NCP][3], cyw[NCP][3];
+  // phase-A columns of this thread
+  uint32_t cx[NCP][3], cy[NCP][3];
   bool cval[NCP];
 #pragma unroll
   for (int k = 0; k < NCP; k++) {
@@ -88,9 +180,10 @@ __global__ void __launch_bounds__(TX* TY) k_fused3(Params<R> p, Bufs<S, R> b, Fu
       cval[k] = gx >= 0 && gy >= 0;
 #pragma unroll
       for (int d = -1; d <= 1; d++) {
-        cxw[k][d + 1] = gx < 0 ? -1 : wrap_coord(gx + d, p.nx, p.periodic[0]);
+        const int xx = gx < 0 ? -1 : wrap_coord(gx + d, p.nx, p.periodic[0]);
         const int yy = gy < 0 ? -1 : wrap_coord(gy + d, p.ny, p.periodic[1]);
-        cyw[k][d + 1] = yy < 0 ? -1 : yy * p.nx;
+        cx[k][d + 1] = (uint32_t)xx * SB;
+        cy[k][d + 1] = (uint32_t)(yy * p.nx) * SB;
       }
     }
   }
@@ -98,11 +191,11 @@ __global__ void __launch_bounds__(TX* TY) k_fused3(Params<R> p, Bufs<S, R> b, Fu
   // phase A: density of plane zz over tile + ring into slot
   auto phase_a = [&](int zz, int slot) {
     const int zc = wrap_coord(zz, p.nz, p.periodic[2]);
-    NbrH<1> cn;
+    RowsB cr;
 #pragma unroll
     for (int d = -1; d <= 1; d++) {
       const int w = zc < 0 ? -1 : wrap_coord(zc + d, p.nz, p.periodic[2]);
-      cn.zw[d + 1] = w < 0 ? -1 : w * p.nxy;
+      cr.z[d + 1] = (uint32_t)(w * p.nxy) * SB;
     }
 #pragma unroll
     for (int k = 0; k < NCP; k++) {
@@ -114,14 +207,14 @@ __global__ void __launch_bounds__(TX* TY) k_fused3(Params<R> p, Bufs<S, R> b, Fu
       if (cval[k] && zc >= 0) {
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-          cn.xw[d] = cxw[k][d];
-          cn.yw[d] = cyw[k][d];
+          cr.x[d] = cx[k][d];
+          cr.y[d] = cy[k][d];
         }
-        const int node = cn.xw[1] + cn.yw[1] + cn.zw[1];
+        const uint32_t node = (cr.x[1] + cr.y[1] + cr.z[1]) / SB;
         const uint32_t mk = b.mask[node];
         if (!(mk & kSelfSolid)) {
           R f[C][Q];
-          pull<3, C, S, R>(p, b.fin, node, mk, cn, f);
+          pull_buf<C, S, R>(p, rin, mk, cr, plane_b, f);
 #pragma unroll
           for (int m = 0; m < C; m++) {
             R s = (R)0;
@@ -155,11 +248,13 @@ __global__ void __launch_bounds__(TX* TY) k_fused3(Params<R> p, Bufs<S, R> b, Fu
     }
     if (active) {
       set_z<3, 1, R>(p, z, nb);
+#pragma unroll
+      for (int d = 0; d < 3; d++) rb.z[d] = (uint32_t)nb.zw[d] * SB;
       const int node = nb.xw[1] + nb.yw[1] + nb.zw[1];
       const uint32_t mk = b.mask[node];
       if (!(mk & kSelfSolid)) {
         R f[C][Q];
-        pull<3, C, S, R>(p, b.fin, node, mk, nb, f);
+        pull_buf<C, S, R>(p, rin, mk, rb, plane_b, f);
         R rs[C];
         if constexpr (RR == 0) {
 #pragma unroll
@@ -189,7 +284,7 @@ __global__ void __launch_bounds__(TX* TY) k_fused3(Params<R> p, Bufs<S, R> b, Fu
         }
         R rl[C], j[C][3], F[C][3], up[3];
         moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
-        collide_store<3, C, FAST, S, R>(p, b.fout, node, f, rl, F, up);
+        collide_store<3, C, FAST, R>(p, BufStore<S, R>{p, rout, (uint32_t)node * SB, plane_b}, f, rl, F, up);
       }
     }
     if constexpr (RR > 0) __syncthreads();

@@ -352,6 +352,15 @@ __device__ __forceinline__ bool is_num(R v) {
   return v == v;
 }
 
+// reciprocal of the FAST arithmetic: the hardware v_rcp_f32 (1 ulp) in FP32
+template <bool FAST, class R>
+__device__ __forceinline__ R recip(R x) {
+  if constexpr (FAST && std::is_same<R, float>::value)
+    return __builtin_amdgcn_rcpf(x);
+  else
+    return (R)1 / x;
+}
+
 // Stencil density in global memory (two-pass step): rho[m*Np + node],
 // NaN on walls, neighbours addressed through the node's rows.
 template <class R, int H>
@@ -426,7 +435,7 @@ __device__ __forceinline__ void ff_force(const Params<R>& p, const A& acc, bool 
   for (int d = 0; d < D; d++)
     if (ws[d] > p.ffeps) {
       if constexpr (FAST) {
-        const R iw = (R)1 / ws[d];
+        const R iw = recip<FAST>(ws[d]);
 #pragma unroll
         for (int m = 0; m < C; m++) {
           R s = (R)0;
@@ -579,7 +588,7 @@ __device__ __forceinline__ void moments_forces(const Params<R>& p, const uint8_t
   R den = (R)0;
 #pragma unroll
   for (int m = 0; m < C; m++) den = den + rl[m] * p.mmot[m];
-  const R iden = (R)1 / den;
+  const R iden = recip<FAST>(den);
 #pragma unroll
   for (int d = 0; d < D; d++) {
     R num = (R)0;
@@ -598,10 +607,9 @@ __device__ __forceinline__ void moments_forces(const Params<R>& p, const uint8_t
 // EXACT (FAST = false) keeps the reference's operations and their order;
 // FAST folds the divisions into reciprocals and the SRT update into
 // f' = (1-w) f + feq (w + pref (1 - w/2)).
-template <int D, int C, bool FAST, class S, class R>
-__device__ __forceinline__ void collide_store(const Params<R>& p, S* __restrict__ fout, int node,
-                                              R (&f)[C][Lat<D>::Q], const R (&rl)[C], const R (&F)[C][3],
-                                              const R (&up)[3]) {
+template <int D, int C, bool FAST, class R, class ST>
+__device__ __forceinline__ void collide_store(const Params<R>& p, const ST& store, R (&f)[C][Lat<D>::Q],
+                                              const R (&rl)[C], const R (&F)[C][3], const R (&up)[3]) {
   using L = Lat<D>;
   constexpr int Q = L::Q;
   R usqr = (R)0;
@@ -613,7 +621,7 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, S* __restrict_
     const R rcs = r * p.cs2[m];
     R feq[Q], pref[Q];
     if constexpr (FAST) {
-      const R irc = (R)1 / rcs;
+      const R irc = recip<FAST>(rcs);
       R Fu = (R)0;
 #pragma unroll
       for (int d = 0; d < D; d++) Fu = Fu + F[m][d] * up[d];
@@ -716,9 +724,20 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, S* __restrict_
       for (int n = 0; n < Q; n++) f[m][n] = f[m][n] + pref[n] * feq[n];
     }
 #pragma unroll
-    for (int n = 0; n < Q; n++) st<S, R>(fout + (size_t)(m * Q + n) * p.Np, node, f[m][n], (R)L::w[n] * p.shift0[m]);
+    for (int n = 0; n < Q; n++) store(m, n, f[m][n]);
   }
 }
+
+// stores f*(node) into the SoA population array of the two-pass kernels
+template <int D, class S, class R>
+struct PtrStore {
+  const Params<R>& p;
+  S* __restrict__ fout;
+  int node;
+  __device__ __forceinline__ void operator()(int m, int n, R v) const {
+    st<S, R>(fout + (size_t)(m * Lat<D>::Q + n) * p.Np, node, v, (R)Lat<D>::w[n] * p.shift0[m]);
+  }
+};
 
 template <int D, int C, int NFF, bool FAST, class S, class R>
 __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int pulled, int apply_bc,
@@ -778,7 +797,7 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
     for (int d = 0; d < D; d++) ex.up[(size_t)d * p.N + node] = (double)up[d];
     return;
   }
-  collide_store<D, C, FAST, S, R>(p, b.fout, node, f, rl, F, up);
+  collide_store<D, C, FAST, R>(p, PtrStore<D, S, R>{p, b.fout, node}, f, rl, F, up);
 }
 
 // FlowFiInit (lbm_flow.F90:923-934): f = (1 - pref/2) feq(rho_IC, u_IC) with
