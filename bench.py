@@ -117,6 +117,8 @@ def main():
     step_gbs = ba * n / ((coll_ms + dens_ms) * 1e-3) / 1e9
     traffic = traffic_for(args.workload)
     masses = dom.reduce()[: d.ncomponents]
+    shape = dom.tune_info()
+    onepass = prof["density"][1] == 0
     out = {
         "metric": "MLUPS",
         "value": round(mlups, 1),
@@ -133,10 +135,13 @@ def main():
         "config": {"workload": args.workload, "description": descr, "lattice": "D3Q19" if d.ndims == 3 else "D2Q9",
                    "nodes": n, "components": d.ncomponents,
                    "collision": "MRT" if d.relaxation_mode == txg.MRT else "BGK",
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   "step_kernel": "one-pass" if onepass else "two-pass",
+                   "tile": [shape[0], shape[1]] if shape and onepass else None,
+                   "zseg": shape[2] if shape and onepass else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "collide pass", "bytes_per_node_step": ba,
+                     "kernel": "one-pass step" if onepass else "collide pass", "bytes_per_node_step": ba,
                      "collide_ms": round(coll_ms, 5), "density_ms": round(dens_ms, 5),
                      "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "mass": [float(x) for x in masses],

@@ -165,14 +165,23 @@ int txg_profile_enable(txg_handle* h, int32_t on);
 int txg_profile_read(txg_handle* h, double* ms, int32_t* n);
 
 /* Step kernel selection (no counterpart in the reference; for tests, the
- * benchmark and the tile autotuner).  kernel: 0 automatic (one-pass where
- * available), 1 two-pass (density pass + collide pass), 2 one-pass
- * (D3Q19: z-marching tiles with the stencil density in LDS).  tile_x x
- * tile_y must be an instantiated tile shape (32x8 or 64x4) or 0 x 0 for
- * the default; zseg is the z-planes per workgroup (0: default).  Returns
- * TXG_ERR_UNSUPPORTED for a kernel or tile that does not exist for the
- * handle's lattice. */
+ * benchmark and the launch-shape autotuner).  kernel: 0 automatic
+ * (one-pass where available), 1 two-pass (density pass + collide pass),
+ * 2 one-pass (D3Q19: z-marching tiles with the stencil density in LDS).
+ * tile_x x tile_y must be an instantiated tile shape (32x8 or 64x4) or
+ * 0 x 0; zseg is the z-planes per workgroup or 0.  A nonzero tile or zseg
+ * fixes the one-pass launch shape (unset parts take their defaults); with
+ * both left at 0 the first one-pass step of the handle runs the autotuner:
+ * it times every tile shape and z-segment length on the live state (a
+ * one-pass launch only reads the current populations, so the state is not
+ * advanced) and keeps the fastest.  Returns TXG_ERR_UNSUPPORTED for a
+ * kernel or tile that does not exist for the handle's lattice. */
 int txg_tune(txg_handle* h, int32_t kernel, int32_t tile_x, int32_t tile_y, int32_t zseg);
+
+/* The one-pass launch shape in use and, when the autotuner chose it, its
+ * time per launch in ms (0 otherwise).  TXG_ERR_STATE before the shape is
+ * fixed (no one-pass step yet and no explicit txg_tune). */
+int txg_tune_info(const txg_handle* h, int32_t* tile_x, int32_t* tile_y, int32_t* zseg, float* ms);
 
 /* Wait for all queued work on the handle's stream. */
 int txg_sync(txg_handle* h);
@@ -181,7 +190,42 @@ int txg_sync(txg_handle* h);
  * NULL), NUL-terminated, truncated to len bytes.  Returns the length. */
 int txg_last_error(const txg_handle* h, char* buf, int32_t len);
 
+/* Destroys h (removing it from its batch first). */
 int txg_destroy(txg_handle* h);
+
+/* ------------------------------------------------------------- batches
+ * Many independent domains advanced together (no counterpart in the
+ * reference, which runs one domain per MPI job; the step each member takes
+ * is exactly txg_step's, so a member's results are bit-identical alone and
+ * in a batch).  Each txg_batch_step launches one persistent batched
+ * one-pass kernel per step over every member's tiles, whatever the members'
+ * sizes.  Members keep their own buffers; the batch orders its launches
+ * after each member's queued work and each member's later work after the
+ * batch, so txg_get_fi, txg_reduce etc. on a member stay valid.  All members
+ * must be D3Q19 with the components, isotropy and precision of the first
+ * member (TXG_ERR_UNSUPPORTED otherwise).  One host thread drives a batch
+ * and its members. */
+typedef struct txg_batch txg_batch;
+
+int txg_batch_create(txg_batch** b);
+/* Adds an initialised handle (txg_init done); a handle is in at most one batch. */
+int txg_batch_add(txg_batch* b, txg_handle* h);
+/* Removes h; it keeps its state and can be stepped alone or added again. */
+int txg_batch_remove(txg_batch* b, txg_handle* h);
+/* Advances every member nsteps steps.  A member fresh from txg_init takes
+ * its first step alone (the initial state is not post-collision), the rest
+ * batched. */
+int txg_batch_step(txg_batch* b, int32_t nsteps);
+int txg_batch_size(const txg_batch* b, int32_t* n);
+/* Launch shape of the batched kernel: tile (32x8 or 64x4) and z-planes per
+ * work item, or 0,0,0 to let the autotuner time the candidates on the live
+ * members at the next step (the default; the state is not advanced). */
+int txg_batch_tune(txg_batch* b, int32_t tile_x, int32_t tile_y, int32_t zseg);
+int txg_batch_tune_info(const txg_batch* b, int32_t* tile_x, int32_t* tile_y, int32_t* zseg, float* ms);
+int txg_batch_sync(txg_batch* b);
+int txg_batch_last_error(const txg_batch* b, char* buf, int32_t len);
+/* Destroys the batch; its members stay valid. */
+int txg_batch_destroy(txg_batch* b);
 
 #ifdef __cplusplus
 }

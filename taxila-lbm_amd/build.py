@@ -30,9 +30,13 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", INC, "-I", CSRC]
 
 UNITS = [
-    # (source, extra flags): the FP64 unit must not fuse multiply-adds
+    # (source, extra flags): the FP64 unit must not fuse multiply-adds; the
+    # FP32 units fuse a*b+c only within one source expression, so a kernel's
+    # rounding does not depend on where it is inlined (tile shape, batched
+    # or single launch, one-pass or two-pass)
     ("txg_fp64.hip", ["-ffp-contract=off"]),
-    ("txg_fp32.hip", ["-ffp-contract=fast"]),
+    ("txg_fp32.hip", ["-ffp-contract=on"]),
+    ("txg_fp16.hip", ["-ffp-contract=on"]),
     ("txg_api.hip", ["-ffp-contract=off"]),
 ]
 
@@ -46,7 +50,7 @@ def _run(cmd):
     return out
 
 
-def build_library(jobs: int = 3) -> str:
+def build_library(jobs: int = 4) -> str:
     os.makedirs(OUT, exist_ok=True)
     objs = []
     cmds = []
@@ -81,7 +85,7 @@ def build_fortran() -> str | None:
     return exe
 
 
-def build(jobs: int = 3) -> None:
+def build(jobs: int = 4) -> None:
     build_library(jobs)
     build_fortran()
 

@@ -112,6 +112,10 @@ inline int cdiv(long a, int b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
 
+namespace {
+void batch_mark_dirty(txg_batch* b);
+}
+
 struct txg_handle {
   txg_desc d;
   int D = 3, Q = 19, C = 1, N = 0, Np = 0;
@@ -149,6 +153,10 @@ struct txg_handle {
   int kernel = 0;       // txg_tune: 0 auto, 1 two-pass, 2 one-pass
   int tile = 0;         // index into kTileX/kTileY
   int zseg = 0;         // z-planes per one-pass workgroup (0: default)
+  int tuned = 0;        // one-pass launch shape fixed (by txg_tune or the autotuner)
+  float tune_ms = 0.f;  // autotuner: time of the chosen shape per launch
+  txg_batch* batch = nullptr;   // batch this handle belongs to (txg_batch_add)
+  hipEvent_t sync_ev = nullptr; // orders the handle's stream against a batch's
   std::string err;
 
   int fail(int code, const char* fmt, ...) {
@@ -182,24 +190,21 @@ Bufs<S, R> make_bufs(txg_handle* h, int in) {
 
 // all kernels take a Bufs by pointer; this holds it for the call
 struct AnyBufs {
-  Bufs<double, double> b64;
-  Bufs<float, float> b32;
-  Bufs<__half, float> b16;
-  const void* ptr = nullptr;
+  int prec = TXG_FP64;
+  Bufs<double, double> b64{};
+  Bufs<float, float> b32{};
+  Bufs<__half, float> b16{};
+  const void* ptr() const {
+    return prec == TXG_FP64 ? (const void*)&b64 : prec == TXG_FP32 ? (const void*)&b32 : (const void*)&b16;
+  }
 };
 
 AnyBufs bufs(txg_handle* h, int in) {
   AnyBufs a;
-  if (h->prec == TXG_FP64) {
-    a.b64 = make_bufs<double, double>(h, in);
-    a.ptr = &a.b64;
-  } else if (h->prec == TXG_FP32) {
-    a.b32 = make_bufs<float, float>(h, in);
-    a.ptr = &a.b32;
-  } else {
-    a.b16 = make_bufs<__half, float>(h, in);
-    a.ptr = &a.b16;
-  }
+  a.prec = h->prec;
+  if (h->prec == TXG_FP64) a.b64 = make_bufs<double, double>(h, in);
+  else if (h->prec == TXG_FP32) a.b32 = make_bufs<float, float>(h, in);
+  else a.b16 = make_bufs<__half, float>(h, in);
   return a;
 }
 
@@ -310,7 +315,7 @@ FusedGeom fused_geom(const txg_handle* h) {
 int ensure_rho(txg_handle* h) {
   if (h->local_state || h->rho_valid) return TXG_OK;
   AnyBufs b = bufs(h, h->cur);
-  int rc = h->hip(h->ks.density(h->params(), b.ptr, 1, 1, h->N, h->stream), "density kernel");
+  int rc = h->hip(h->ks.density(h->params(), b.ptr(), 1, 1, h->N, h->stream), "density kernel");
   if (rc) return rc;
   h->rho_valid = 1;
   return TXG_OK;
@@ -338,7 +343,7 @@ int run_export(txg_handle* h) {
   ExportPtrs e = export_ptrs(h);
   AnyBufs b = bufs(h, h->cur);
   int pulled = h->local_state ? 0 : 1;
-  rc = h->hip(h->ks.collide(h->params(), b.ptr, pulled, pulled, &e, h->N, h->stream), "export kernel");
+  rc = h->hip(h->ks.collide(h->params(), b.ptr(), pulled, pulled, &e, h->N, h->stream), "export kernel");
   if (rc) return rc;
   return h->hip(hipStreamSynchronize(h->stream), "export sync");
 }
@@ -465,7 +470,10 @@ int txg_create(const txg_desc* d, txg_handle** out) {
     } else {
       std::vector<float> s((size_t)h->C * Q, 0.0f);
       for (int m = 0; m < h->C; m++)
-        for (int n = 0; n < Q; n++) s[m * Q + n] = (float)wq[n] * h->p32.shift0[m];
+        for (int n = 0; n < Q; n++) {
+          s[m * Q + n] = (float)wq[n] * h->p32.shift0[m];
+          h->p32.shift[m][n] = s[m * Q + n];
+        }
       (void)hipMemcpy(h->shift, s.data(), s.size() * 4, hipMemcpyHostToDevice);
     }
   }
@@ -648,17 +656,18 @@ int txg_set_fi(txg_handle* h, const double* fi) {
 int txg_init(txg_handle* h) {
   if (!h) return TXG_ERR_ARG;
   if (!h->walls_set || !h->state_set) return h->fail(TXG_ERR_STATE, "set walls and state before txg_init");
+  if (h->batch) batch_mark_dirty(h->batch);
   int rc;
   if (!h->fi_set) {
     // FlowFiInit: f_0 into buffer 0 (the kernel writes b.fout = f[1 - in])
     AnyBufs b = bufs(h, 1);
-    rc = h->hip(h->ks.fiinit(h->params(), b.ptr, h->u_ic, h->N, h->stream), "fiinit kernel");
+    rc = h->hip(h->ks.fiinit(h->params(), b.ptr(), h->u_ic, h->N, h->stream), "fiinit kernel");
     if (rc) return rc;
     h->cur = 0;
   }
   // FlowUpdateMoments: rho = sum f (no boundary conditions at start-up)
   AnyBufs b = bufs(h, h->cur);
-  rc = h->hip(h->ks.density(h->params(), b.ptr, 0, 0, h->N, h->stream), "density kernel");
+  rc = h->hip(h->ks.density(h->params(), b.ptr(), 0, 0, h->N, h->stream), "density kernel");
   if (rc) return rc;
   h->local_state = 1;
   h->rho_valid = 1;
@@ -695,19 +704,87 @@ struct ProfScope {
 };
 }  // namespace
 
+namespace {
+// One-pass launch-shape autotuner.  A one-pass launch reads f[cur] and
+// writes the other buffer only, so repeating it leaves the state as it was:
+// every candidate (tile shape x z-planes per workgroup) is launched on the
+// live state, once untimed and `reps` times between events, and the fastest
+// mean wins.  Candidates keep at least 128 workgroups (half a workgroup per
+// CU) unless the domain is too small to reach that with 4-plane segments.
+int autotune(txg_handle* h, int reps) {
+  const int nz = h->d.nz;
+  static const int kZ[] = {4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+  hipEvent_t e0, e1;
+  int rc = h->hip(hipEventCreate(&e0), "tune event");
+  if (rc) return rc;
+  rc = h->hip(hipEventCreate(&e1), "tune event");
+  if (rc) {
+    (void)hipEventDestroy(e0);
+    return rc;
+  }
+  AnyBufs b = bufs(h, h->cur);
+  float best = 0.f;
+  int best_tile = h->tile, best_z = h->zseg;
+  for (int t = 0; t < kNumTiles && !rc; t++) {
+    if (!h->ks.fused[t]) continue;
+    h->tile = t;
+    int max_blocks = 0;
+    for (int zi = 0; zi < (int)(sizeof(kZ) / sizeof(kZ[0])); zi++) {
+      h->zseg = std::min(kZ[zi], nz);
+      max_blocks = std::max(max_blocks, fused_geom(h).nblocks);
+    }
+    int last = -1;
+    for (int zi = 0; zi < (int)(sizeof(kZ) / sizeof(kZ[0])) && !rc; zi++) {
+      const int z = std::min(kZ[zi], nz);
+      if (z == last) break;
+      last = z;
+      h->zseg = z;
+      const FusedGeom g = fused_geom(h);
+      if (g.nblocks < std::min(128, max_blocks)) continue;
+      rc = h->hip(h->ks.fused[t](h->params(), b.ptr(), &g, h->stream), "one-pass kernel (tuning)");
+      if (!rc) rc = h->hip(hipEventRecord(e0, h->stream), "tune event");
+      for (int r = 0; r < reps && !rc; r++)
+        rc = h->hip(h->ks.fused[t](h->params(), b.ptr(), &g, h->stream), "one-pass kernel (tuning)");
+      if (!rc) rc = h->hip(hipEventRecord(e1, h->stream), "tune event");
+      if (!rc) rc = h->hip(hipEventSynchronize(e1), "tune sync");
+      float ms = 0.f;
+      if (!rc) rc = h->hip(hipEventElapsedTime(&ms, e0, e1), "tune timing");
+      ms /= reps;
+      if (!rc && (best == 0.f || ms < best)) {
+        best = ms;
+        best_tile = t;
+        best_z = z;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->tile = best_tile;
+  h->zseg = best_z;
+  h->tune_ms = best;
+  h->tuned = 1;
+  return rc;
+}
+}  // namespace
+
 int txg_step(txg_handle* h, int32_t nsteps) {
   if (!h || nsteps < 0) return TXG_ERR_ARG;
   if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
+  if (h->batch && nsteps > 0) batch_mark_dirty(h->batch);  // the batch's orientation snapshot goes stale
   const bool fused = use_fused(h);
-  const FusedGeom g = fused ? fused_geom(h) : FusedGeom{};
   for (int s = 0; s < nsteps; s++) {
     int rc;
+    if (fused && !h->local_state && !h->tuned) {
+      rc = autotune(h, 3);
+      if (rc) return rc;
+    }
     if (fused && !h->local_state) {
+      const FusedGeom g = fused_geom(h);
       // one pass: pull, density (LDS), forces, collide
       AnyBufs b = bufs(h, h->cur);
       {
         ProfScope ps(h, 0);
-        rc = h->hip(h->ks.fused[h->tile](h->params(), b.ptr, &g, h->stream), "one-pass kernel");
+        rc = h->hip(h->ks.fused[h->tile](h->params(), b.ptr(), &g, h->stream), "one-pass kernel");
       }
       if (rc) return rc;
       h->cur = 1 - h->cur;
@@ -720,7 +797,7 @@ int txg_step(txg_handle* h, int32_t nsteps) {
     int pulled = h->local_state ? 0 : 1;
     {
       ProfScope ps(h, 0);
-      rc = h->hip(h->ks.collide(h->params(), b.ptr, pulled, pulled, nullptr, h->N, h->stream), "collide kernel");
+      rc = h->hip(h->ks.collide(h->params(), b.ptr(), pulled, pulled, nullptr, h->N, h->stream), "collide kernel");
     }
     if (rc) return rc;
     h->cur = 1 - h->cur;
@@ -730,7 +807,7 @@ int txg_step(txg_handle* h, int32_t nsteps) {
     AnyBufs b2 = bufs(h, h->cur);
     {
       ProfScope ps(h, 1);
-      rc = h->hip(h->ks.density(h->params(), b2.ptr, 1, 1, h->N, h->stream), "density kernel");
+      rc = h->hip(h->ks.density(h->params(), b2.ptr(), 1, 1, h->N, h->stream), "density kernel");
     }
     if (rc) return rc;
     h->rho_valid = 1;
@@ -756,7 +833,19 @@ int txg_tune(txg_handle* h, int32_t kernel, int32_t tile_x, int32_t tile_y, int3
   h->kernel = kernel;
   h->tile = tile;
   h->zseg = zseg;
+  // an explicit tile or z-segment fixes the one-pass shape; none re-arms the
+  // autotuner for the next one-pass step
+  h->tuned = (tile_x != 0 || tile_y != 0 || zseg != 0) ? 1 : 0;
   return TXG_OK;
+}
+
+int txg_tune_info(const txg_handle* h, int32_t* tile_x, int32_t* tile_y, int32_t* zseg, float* ms) {
+  if (!h || !tile_x || !tile_y || !zseg || !ms) return TXG_ERR_ARG;
+  *tile_x = kTileX[h->tile];
+  *tile_y = kTileY[h->tile];
+  *zseg = h->zseg;
+  *ms = h->tune_ms;
+  return h->tuned ? TXG_OK : TXG_ERR_STATE;
 }
 
 int txg_profile_enable(txg_handle* h, int32_t on) {
@@ -938,7 +1027,9 @@ int txg_last_error(const txg_handle* h, char* buf, int32_t len) {
 
 int txg_destroy(txg_handle* h) {
   if (!h) return TXG_OK;
+  if (h->batch) (void)txg_batch_remove(h->batch, h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->sync_ev) (void)hipEventDestroy(h->sync_ev);
   (void)hipFree(h->f[0]);
   (void)hipFree(h->f[1]);
   (void)hipFree(h->rho);
@@ -952,6 +1043,370 @@ int txg_destroy(txg_handle* h) {
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
+  return TXG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+// A batch advances many independent handles (porous samples, requests) with
+// one batched one-pass launch per step.  Members keep their own device
+// buffers and streams; the batch stream waits for every member's stream
+// before its launches and every member's stream waits for the batch after
+// them, so per-handle calls stay ordered.  All members share the kernel
+// configuration (D3Q19, components, isotropy, precision) of the first one.
+struct txg_batch {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<txg_handle*> mem;
+  KernelSet ks{};
+  int C = 0, iso = -1, prec = -1;
+  // device member table of the current launch set
+  void* dmem = nullptr;
+  size_t dmem_cap = 0;
+  int* dprefix = nullptr;
+  size_t dprefix_cap = 0;
+  std::vector<txg_handle*> table;  // members in the device table, in order
+  int table_dirty = 1;
+  int flip = 0;  // batched steps since the table was written, mod 2
+  int grid = 0;
+  int tile = 0, zseg = 0, tuned = 0;
+  float tune_ms = 0.f;
+  std::string err;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+  int hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return TXG_OK;
+    return fail(TXG_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  }
+};
+
+namespace {
+
+void batch_mark_dirty(txg_batch* b) { b->table_dirty = 1; }
+
+int batch_zseg(const txg_batch* b, const txg_handle* h) { return std::min(b->zseg > 0 ? b->zseg : 16, h->d.nz); }
+
+template <class S, class R>
+void fill_member(const txg_batch* b, txg_handle* h, const Params<R>& p, BatchMember<S, R>& m) {
+  m.p = p;
+  m.b[0] = make_bufs<S, R>(h, 0);
+  m.b[1] = make_bufs<S, R>(h, 1);
+  const int tx = kTileX[b->tile], ty = kTileY[b->tile];
+  FusedGeom& g = m.g;
+  g.ntx = (h->d.nx + tx - 1) / tx;
+  g.nty = (h->d.ny + ty - 1) / ty;
+  g.ntiles = g.ntx * g.nty;
+  g.zseg = batch_zseg(b, h);
+  g.nseg = (h->d.nz + g.zseg - 1) / g.zseg;
+  g.nblocks = g.ntiles * g.nseg;
+  m.parity = h->cur;
+}
+
+// write the device table for `set` (members in pulled state); flip restarts
+int write_table(txg_batch* b, const std::vector<txg_handle*>& set) {
+  const size_t mb = b->ks.member_bytes;
+  std::vector<unsigned char> host(mb * std::max<size_t>(1, set.size()));
+  std::vector<int> prefix(set.size() + 1, 0);
+  for (size_t i = 0; i < set.size(); i++) {
+    txg_handle* h = set[i];
+    void* dst = host.data() + i * mb;
+    int nblocks;
+    if (b->prec == TXG_FP64) {
+      auto* m = new (dst) BatchMember<double, double>();
+      fill_member<double, double>(b, h, h->p64, *m);
+      nblocks = m->g.nblocks;
+    } else if (b->prec == TXG_FP32) {
+      auto* m = new (dst) BatchMember<float, float>();
+      fill_member<float, float>(b, h, h->p32, *m);
+      nblocks = m->g.nblocks;
+    } else {
+      auto* m = new (dst) BatchMember<__half, float>();
+      fill_member<__half, float>(b, h, h->p32, *m);
+      nblocks = m->g.nblocks;
+    }
+    prefix[i + 1] = prefix[i] + nblocks;
+  }
+  // the previous launches may still read the old table
+  int rc = b->hip(hipStreamSynchronize(b->stream), "batch sync");
+  if (rc) return rc;
+  if (host.size() > b->dmem_cap) {
+    (void)hipFree(b->dmem);
+    b->dmem = nullptr;
+    b->dmem_cap = 0;
+    if (hipMalloc(&b->dmem, host.size()) != hipSuccess) return b->fail(TXG_ERR_NOMEM, "batch table allocation");
+    b->dmem_cap = host.size();
+  }
+  if (prefix.size() * sizeof(int) > b->dprefix_cap) {
+    (void)hipFree(b->dprefix);
+    b->dprefix = nullptr;
+    b->dprefix_cap = 0;
+    if (hipMalloc(&b->dprefix, prefix.size() * sizeof(int)) != hipSuccess)
+      return b->fail(TXG_ERR_NOMEM, "batch table allocation");
+    b->dprefix_cap = prefix.size() * sizeof(int);
+  }
+  rc = b->hip(hipMemcpy(b->dmem, host.data(), host.size(), hipMemcpyHostToDevice), "batch table upload");
+  if (!rc) rc = b->hip(hipMemcpy(b->dprefix, prefix.data(), prefix.size() * sizeof(int), hipMemcpyHostToDevice),
+                       "batch table upload");
+  if (rc) return rc;
+  b->table = set;
+  b->flip = 0;
+  const int items = prefix.back();
+  int occ = b->ks.fused_batch_occupancy[b->tile] ? b->ks.fused_batch_occupancy[b->tile]() : 0;
+  int cus = 0;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  b->grid = std::max(1, std::min(items, std::max(1, occ) * cus));
+  return TXG_OK;
+}
+
+int batch_launch(txg_batch* b) {
+  if (b->table.empty()) return TXG_OK;
+  return b->hip(b->ks.fused_batch[b->tile](b->dmem, b->dprefix, (int)b->table.size(), b->flip, b->grid, b->stream),
+                "batched one-pass kernel");
+}
+
+// after a batched launch: host bookkeeping of every member in the table
+void batch_advance(txg_batch* b) {
+  b->flip ^= 1;
+  for (txg_handle* h : b->table) {
+    h->cur = 1 - h->cur;
+    h->rho_valid = 0;
+  }
+}
+
+// the batch stream waits for every member's queued work
+int batch_acquire(txg_batch* b) {
+  for (txg_handle* h : b->mem) {
+    if (!h->sync_ev && hipEventCreateWithFlags(&h->sync_ev, hipEventDisableTiming) != hipSuccess)
+      return b->fail(TXG_ERR_HIP, "event creation");
+    int rc = b->hip(hipEventRecord(h->sync_ev, h->stream), "member event");
+    if (!rc) rc = b->hip(hipStreamWaitEvent(b->stream, h->sync_ev, 0), "member wait");
+    if (rc) return rc;
+  }
+  return TXG_OK;
+}
+
+// every member's stream waits for the batch's launches
+int batch_release(txg_batch* b) {
+  int rc = b->hip(hipEventRecord(b->done, b->stream), "batch event");
+  for (txg_handle* h : b->mem)
+    if (!rc) rc = b->hip(hipStreamWaitEvent(h->stream, b->done, 0), "batch wait");
+  return rc;
+}
+
+// time the candidate launch shapes on the live state (the launch reads
+// f[cur] of every member and writes the other buffer, so repeating it with
+// the same flip does not advance anything) and keep the fastest
+int batch_autotune(txg_batch* b, const std::vector<txg_handle*>& set) {
+  static const int kZ[] = {4, 8, 12, 16, 24, 32, 48, 64};
+  hipEvent_t e0, e1;
+  int rc = b->hip(hipEventCreate(&e0), "tune event");
+  if (rc) return rc;
+  rc = b->hip(hipEventCreate(&e1), "tune event");
+  if (rc) {
+    (void)hipEventDestroy(e0);
+    return rc;
+  }
+  float best = 0.f;
+  int bt = 0, bz = 16;
+  for (int t = 0; t < kNumTiles && !rc; t++) {
+    if (!b->ks.fused_batch[t]) continue;
+    for (int zi = 0; zi < (int)(sizeof(kZ) / sizeof(kZ[0])) && !rc; zi++) {
+      b->tile = t;
+      b->zseg = kZ[zi];
+      rc = write_table(b, set);
+      if (!rc) rc = batch_launch(b);
+      if (!rc) rc = b->hip(hipEventRecord(e0, b->stream), "tune event");
+      for (int r = 0; r < 2 && !rc; r++) rc = batch_launch(b);
+      if (!rc) rc = b->hip(hipEventRecord(e1, b->stream), "tune event");
+      if (!rc) rc = b->hip(hipEventSynchronize(e1), "tune sync");
+      float ms = 0.f;
+      if (!rc) rc = b->hip(hipEventElapsedTime(&ms, e0, e1), "tune timing");
+      ms *= 0.5f;
+      if (!rc && (best == 0.f || ms < best)) {
+        best = ms;
+        bt = t;
+        bz = kZ[zi];
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tile = bt;
+  b->zseg = bz;
+  b->tune_ms = best;
+  b->tuned = 1;
+  b->table_dirty = 1;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int txg_batch_create(txg_batch** out) {
+  if (!out) return TXG_ERR_ARG;
+  *out = nullptr;
+  txg_batch* b = new txg_batch();
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return TXG_ERR_HIP;
+  }
+  *out = b;
+  return TXG_OK;
+}
+
+int txg_batch_add(txg_batch* b, txg_handle* h) {
+  if (!b || !h) return TXG_ERR_ARG;
+  if (h->batch) return b->fail(TXG_ERR_STATE, "handle already belongs to a batch");
+  if (!h->initialized) return b->fail(TXG_ERR_STATE, "txg_init has not been called on the handle");
+  if (h->D != 3 || !h->ks.fused_batch[0])
+    return b->fail(TXG_ERR_UNSUPPORTED, "batched steps need a D3Q19 handle");
+  if ((uint64_t)h->Q * h->Np * h->ssize >= (1ull << 32))
+    return b->fail(TXG_ERR_UNSUPPORTED, "batched steps need < 4 GiB of populations per component");
+  const int iso = h->d.isotropy_order;
+  if (b->mem.empty()) {
+    b->ks = h->ks;
+    b->C = h->C;
+    b->iso = iso;
+    b->prec = h->prec;
+  } else if (h->C != b->C || iso != b->iso || h->prec != b->prec) {
+    return b->fail(TXG_ERR_UNSUPPORTED, "batch members must share components, isotropy and precision");
+  }
+  b->mem.push_back(h);
+  h->batch = b;
+  b->table_dirty = 1;
+  return TXG_OK;
+}
+
+int txg_batch_remove(txg_batch* b, txg_handle* h) {
+  if (!b || !h) return TXG_ERR_ARG;
+  auto it = std::find(b->mem.begin(), b->mem.end(), h);
+  if (it == b->mem.end() || h->batch != b) return b->fail(TXG_ERR_ARG, "handle is not a member of this batch");
+  // the handle's stream already waits for the batch's launches (batch_release)
+  b->mem.erase(it);
+  h->batch = nullptr;
+  b->table_dirty = 1;
+  return TXG_OK;
+}
+
+int txg_batch_step(txg_batch* b, int32_t nsteps) {
+  if (!b || nsteps < 0) return TXG_ERR_ARG;
+  if (b->mem.empty() || nsteps == 0) return TXG_OK;
+  int rc = batch_acquire(b);
+  if (rc) return rc;
+  // members fresh from txg_init hold local populations: their first step is
+  // their own (two-pass) step, beside the batched step of the others
+  std::vector<txg_handle*> pulled, local;
+  for (txg_handle* h : b->mem) (h->local_state ? local : pulled).push_back(h);
+  int first = 0;
+  if (!local.empty()) {
+    for (txg_handle* h : local) {
+      rc = txg_step(h, 1);
+      if (rc) return b->fail(rc, "member step: %s", h->err.c_str());
+    }
+    if (!pulled.empty()) {
+      if (!b->tuned) rc = batch_autotune(b, pulled);
+      if (!rc) rc = write_table(b, pulled);
+      if (!rc) rc = batch_launch(b);
+      if (rc) return rc;
+      batch_advance(b);
+    }
+    b->table_dirty = 1;
+    first = 1;
+    rc = batch_acquire(b);  // the members' own first steps
+    if (rc) return rc;
+  }
+  if (first < nsteps) {
+    if (!b->tuned) {
+      rc = batch_autotune(b, b->mem);
+      if (rc) return rc;
+    }
+    if (b->table_dirty || b->table.size() != b->mem.size()) {
+      rc = write_table(b, b->mem);
+      if (rc) return rc;
+      b->table_dirty = 0;
+    }
+    for (int s = first; s < nsteps; s++) {
+      rc = batch_launch(b);
+      if (rc) return rc;
+      batch_advance(b);
+    }
+  }
+  return batch_release(b);
+}
+
+int txg_batch_size(const txg_batch* b, int32_t* n) {
+  if (!b || !n) return TXG_ERR_ARG;
+  *n = (int32_t)b->mem.size();
+  return TXG_OK;
+}
+
+int txg_batch_tune(txg_batch* b, int32_t tile_x, int32_t tile_y, int32_t zseg) {
+  if (!b || zseg < 0) return TXG_ERR_ARG;
+  int rc = b->hip(hipStreamSynchronize(b->stream), "batch sync");
+  if (rc) return rc;
+  if (tile_x == 0 && tile_y == 0 && zseg == 0) {
+    b->tuned = 0;
+    b->table_dirty = 1;
+    return TXG_OK;
+  }
+  int tile = b->tile;
+  if (tile_x != 0 || tile_y != 0) {
+    tile = -1;
+    for (int i = 0; i < kNumTiles; i++)
+      if (kTileX[i] == tile_x && kTileY[i] == tile_y) tile = i;
+    if (tile < 0) return b->fail(TXG_ERR_UNSUPPORTED, "tile %dx%d is not instantiated", tile_x, tile_y);
+  }
+  b->tile = tile;
+  b->zseg = zseg;
+  b->tuned = 1;
+  b->tune_ms = 0.f;
+  b->table_dirty = 1;
+  return TXG_OK;
+}
+
+int txg_batch_tune_info(const txg_batch* b, int32_t* tile_x, int32_t* tile_y, int32_t* zseg, float* ms) {
+  if (!b || !tile_x || !tile_y || !zseg || !ms) return TXG_ERR_ARG;
+  *tile_x = kTileX[b->tile];
+  *tile_y = kTileY[b->tile];
+  *zseg = b->zseg > 0 ? b->zseg : 16;
+  *ms = b->tune_ms;
+  return b->tuned ? TXG_OK : TXG_ERR_STATE;
+}
+
+int txg_batch_sync(txg_batch* b) {
+  if (!b) return TXG_ERR_ARG;
+  return b->hip(hipStreamSynchronize(b->stream), "batch sync");
+}
+
+int txg_batch_last_error(const txg_batch* b, char* buf, int32_t len) {
+  if (!b || !buf || len <= 0) return 0;
+  snprintf(buf, (size_t)len, "%s", b->err.c_str());
+  return (int)std::min<size_t>(b->err.size(), (size_t)len - 1);
+}
+
+int txg_batch_destroy(txg_batch* b) {
+  if (!b) return TXG_OK;
+  (void)hipStreamSynchronize(b->stream);
+  for (txg_handle* h : b->mem) h->batch = nullptr;
+  (void)hipFree(b->dmem);
+  (void)hipFree(b->dprefix);
+  (void)hipEventDestroy(b->done);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
   return TXG_OK;
 }
 

@@ -29,6 +29,13 @@ struct KernelSet {
   hipError_t (*fiinit)(const void* p, const void* b, const void* u_ic, int N, hipStream_t s);
   // one-pass step (D3Q19 only; null otherwise), per tile shape
   hipError_t (*fused[kNumTiles])(const void* p, const void* b, const FusedGeom* g, hipStream_t s);
+  // batched one-pass step over nmem members (mem: device BatchMember<S,R>[nmem],
+  // prefix: device int[nmem+1]) on a persistent grid of `grid` workgroups
+  hipError_t (*fused_batch[kNumTiles])(const void* mem, const int* prefix, int nmem, int flip, int grid,
+                                       hipStream_t s);
+  // workgroups of the batched kernel resident per CU
+  int (*fused_batch_occupancy[kNumTiles])();
+  size_t member_bytes;  // sizeof(BatchMember<S,R>)
 };
 
 // returns false if the configuration is not instantiated

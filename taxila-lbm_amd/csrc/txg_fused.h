@@ -7,13 +7,18 @@
 // the workgroup
 //   A. pull-sums the density of plane z+r over tile+ring into an LDS ring
 //      buffer of 2r+1 planes (NaN marks walls and cells beyond non-periodic
-//      faces),
-//   B. re-pulls its TX x TY interior nodes of plane z (read r planes ago by
-//      the same workgroup, so served from L2/MALL) and runs moments, forces
-//      (stencil density from LDS), common velocity and collision, writing f*.
-// HBM therefore sees the populations read once and written once per step,
-// versus read twice for the two-pass step.  Without a fluid-fluid force
-// (r = 0) phase A disappears and the density is the node's own.
+//      faces); each thread pulls its own interior column and the ring columns
+//      are spread over the threads,
+//   B. runs moments, forces (stencil density from LDS), common velocity and
+//      collision on its interior node of plane z, writing f*.
+// The populations of the own column pulled in phase A are kept in a
+// register ring of r+1 planes and collided r planes later, so each stored
+// population is fetched from HBM once; only ring columns, which the
+// neighbouring tile also reads, rely on L2.  Where r+1 planes of C x 19
+// values do not fit in registers (two components at isotropy 8, FP64 with
+// two components), phase B re-pulls plane z instead, read r planes earlier
+// by the same workgroup.  Without a fluid-fluid force (r = 0) phase A
+// disappears and the density is the node's own.
 //
 // Populations are addressed through one buffer resource per component
 // (base in SGPRs, 32-bit byte offsets per lane, hardware range check), so a
@@ -50,12 +55,19 @@ __device__ __forceinline__ R bld(rsrc_t r, uint32_t off, R shift) {
   }
 }
 
+// FP32/FP64 stores of f* are non-temporal (cache policy nt): nothing reads
+// them again within the step, and streaming them past the caches measured
+// 10-15% more HBM bandwidth than default-policy stores for this 38-array SoA
+// pattern.  16-bit stores keep the default policy (nt made the FP16-storage
+// step several times slower).
+constexpr int kStoreNT = 2;
+
 template <class S, class R>
 __device__ __forceinline__ void bst(rsrc_t r, uint32_t off, R v, R shift) {
   if constexpr (std::is_same<S, float>::value) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, kStoreNT);
   } else if constexpr (std::is_same<S, double>::value) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)off, 0, kStoreNT);
   } else {
     __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn((float)(v - shift))), r, (int)off, 0, 0);
   }
@@ -84,7 +96,7 @@ __device__ __forceinline__ void pull_buf(const Params<R>& p, const rsrc_t (&rin)
       off = (mk & (1u << n)) ? bo : so;
     }
 #pragma unroll
-    for (int m = 0; m < C; m++) f[m][n] = bld<S, R>(rin[m], off, (R)L::w[n] * p.shift0[m]);
+    for (int m = 0; m < C; m++) f[m][n] = bld<S, R>(rin[m], off, p.shift[m][n]);
   });
 }
 
@@ -94,7 +106,7 @@ struct BufStore {
   const rsrc_t* rout;
   uint32_t node_b, plane_b;
   __device__ __forceinline__ void operator()(int m, int n, R v) const {
-    bst<S, R>(rout[m], node_b + (uint32_t)n * plane_b, v, (R)Lat<3>::w[n] * p.shift0[m]);
+    bst<S, R>(rout[m], node_b + (uint32_t)n * plane_b, v, p.shift[m][n]);
   }
 };
 
@@ -127,16 +139,59 @@ __device__ __forceinline__ int xcd_remap(int i, int nblocks) {
   return xcd * per + min(xcd, rem) + k;
 }
 
+// ring column q of the extended tile (the cells outside the TX x TY interior)
+template <int TX, int TY, int RR>
+__device__ __forceinline__ void ring_col(int q, int& cx, int& cy) {
+  constexpr int EXW = TX + 2 * RR;
+  if (q < RR * EXW) {
+    cy = q / EXW;
+    cx = q % EXW;
+    return;
+  }
+  q -= RR * EXW;
+  if (q < RR * EXW) {
+    cy = RR + TY + q / EXW;
+    cx = q % EXW;
+    return;
+  }
+  q -= RR * EXW;
+  constexpr int SIDE = RR > 0 ? 2 * RR : 1;  // ring columns per middle row
+  cy = RR + q / SIDE;
+  const int k = q % SIDE;
+  cx = k < RR ? k : TX + k;
+}
+
+#ifndef TXG_FUSED_WAVES_PER_EU
+#define TXG_FUSED_WAVES_PER_EU 1
+#endif
+
+// stencil geometry of the one-pass kernel for isotropy stencil NFF
+template <int NFF, int TX, int TY>
+struct FusedShape {
+  static constexpr int RR = NFF == kFF3Count8 ? 2 : (NFF == kFF3Count4 ? 1 : 0);
+  static constexpr int EXW = TX + 2 * RR, EYW = TY + 2 * RR, NCOL = EXW * EYW, NPL = 2 * RR + 1;
+  template <int C>
+  static constexpr int lds_len() { return RR > 0 ? NPL * C * NCOL : 1; }
+};
+
+// one work item of the one-pass step: the TX x TY column tile at (x0, y0)
+// over planes z0..z1-1.  srho is the workgroup's LDS density ring.  Ends
+// with a workgroup barrier when it uses LDS, so items can follow each other.
 template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_fused3(Params<R> p, Bufs<S, R> b, FusedGeom g) {
+__device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>& b, int x0, int y0, int z0, int z1,
+                                            R* srho) {
   using L = Lat<3>;
+  using Sh = FusedShape<NFF, TX, TY>;
   constexpr int Q = L::Q;
-  constexpr int RR = NFF == kFF3Count8 ? 2 : (NFF == kFF3Count4 ? 1 : 0);
-  constexpr int EXW = TX + 2 * RR, EYW = TY + 2 * RR, NCOL = EXW * EYW, NPL = 2 * RR + 1;
+  constexpr int RR = Sh::RR;
+  constexpr int EXW = Sh::EXW, EYW = Sh::EYW, NCOL = Sh::NCOL, NPL = Sh::NPL;
   constexpr int NT = TX * TY;
-  constexpr int NCP = (NCOL + NT - 1) / NT;  // columns per thread in phase A
+  constexpr int NRING = NCOL - NT;
+  constexpr int NRP = (NRING + NT - 1) / NT;  // ring columns per thread
   constexpr uint32_t SB = sizeof(S);
-  __shared__ R srho[RR > 0 ? NPL * C * NCOL : 1];
+  // keep the own column's pulled populations of planes z..z+RR in registers
+  // when (RR+1)*C*Q values fit beside the collision's working set
+  constexpr bool KEEP = (RR + 1) * C * (int)sizeof(R) <= 16;
 
   const uint32_t plane_b = (uint32_t)p.Np * SB;
   rsrc_t rin[C], rout[C];
@@ -146,148 +201,293 @@ __global__ void __launch_bounds__(TX* TY) k_fused3(Params<R> p, Bufs<S, R> b, Fu
     rout[m] = make_rsrc(b.fout + (size_t)m * Q * p.Np, (uint32_t)Q * plane_b);
   }
 
-  const int lb = xcd_remap(blockIdx.x, g.nblocks);
-  const int tile = lb % g.ntiles, seg = lb / g.ntiles;
-  const int x0 = (tile % g.ntx) * TX, y0 = (tile / g.ntx) * TY;
-  const int z0 = seg * g.zseg, z1 = min(p.nz, z0 + g.zseg);
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
   const int x = x0 + tx, y = y0 + ty;
   const bool active = x < p.nx && y < p.ny;
 
-  // rows of the interior column; nb (element units) serves the wettability force
-  RowsB rb;
-  NbrH<1> nb;
-  if (active) {
-    make_nbr<3, 1, R>(p, x, y, z0, nb);
+  // rows of a column at (gx, gy) (wrapped), +-1 in x and y, in bytes
+  auto col_rows = [&](int cxg, int cyg, uint32_t (&xr)[3], uint32_t (&yr)[3]) -> bool {
+    const int gx = wrap_coord(cxg, p.nx, p.periodic[0]);
+    const int gy = wrap_coord(cyg, p.ny, p.periodic[1]);
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-      rb.x[d] = (uint32_t)nb.xw[d] * SB;
-      rb.y[d] = (uint32_t)nb.yw[d] * SB;
+    for (int d = -1; d <= 1; d++) {
+      const int xx = gx < 0 ? -1 : wrap_coord(gx + d, p.nx, p.periodic[0]);
+      const int yy = gy < 0 ? -1 : wrap_coord(gy + d, p.ny, p.periodic[1]);
+      xr[d + 1] = (uint32_t)xx * SB;
+      yr[d + 1] = (uint32_t)(yy * p.nx) * SB;
+    }
+    return gx >= 0 && gy >= 0;
+  };
+
+  // own column (the thread's interior position; wrapped for threads past the
+  // domain edge, whose column is still a stencil neighbour)
+  uint32_t ox[3], oy[3];
+  const bool oval = col_rows(x, y, ox, oy);
+  const int ocol = (ty + RR) * EXW + tx + RR;
+  // ring columns
+  uint32_t rx[NRP > 0 ? NRP : 1][3], ry[NRP > 0 ? NRP : 1][3];
+  bool rval[NRP > 0 ? NRP : 1];
+  int rcol[NRP > 0 ? NRP : 1];
+#pragma unroll
+  for (int k = 0; k < NRP; k++) {
+    const int q = t + k * NT;
+    rval[k] = false;
+    rcol[k] = -1;
+    if (q < NRING) {
+      int cxx, cyy;
+      ring_col<TX, TY, RR>(q, cxx, cyy);
+      rcol[k] = cyy * EXW + cxx;
+      rval[k] = col_rows(x0 - RR + cxx, y0 - RR + cyy, rx[k], ry[k]);
     }
   }
+  NbrH<1> nb;  // element rows of the own column (wettability force)
+  if (active) make_nbr<3, 1, R>(p, x, y, z0, nb);
 
-  // phase-A columns of this thread
-  uint32_t cx[NCP][3], cy[NCP][3];
-  bool cval[NCP];
-#pragma unroll
-  for (int k = 0; k < NCP; k++) {
-    const int c = t + k * NT;
-    cval[k] = false;
-    if (RR > 0 && c < NCOL) {
-      const int gx = wrap_coord(x0 - RR + c % EXW, p.nx, p.periodic[0]);
-      const int gy = wrap_coord(y0 - RR + c / EXW, p.ny, p.periodic[1]);
-      cval[k] = gx >= 0 && gy >= 0;
-#pragma unroll
-      for (int d = -1; d <= 1; d++) {
-        const int xx = gx < 0 ? -1 : wrap_coord(gx + d, p.nx, p.periodic[0]);
-        const int yy = gy < 0 ? -1 : wrap_coord(gy + d, p.ny, p.periodic[1]);
-        cx[k][d + 1] = (uint32_t)xx * SB;
-        cy[k][d + 1] = (uint32_t)(yy * p.nx) * SB;
-      }
-    }
-  }
-
-  // phase A: density of plane zz over tile + ring into slot
-  auto phase_a = [&](int zz, int slot) {
+  auto zrows = [&](int zz, uint32_t (&zr)[3]) -> bool {
     const int zc = wrap_coord(zz, p.nz, p.periodic[2]);
-    RowsB cr;
 #pragma unroll
     for (int d = -1; d <= 1; d++) {
       const int w = zc < 0 ? -1 : wrap_coord(zc + d, p.nz, p.periodic[2]);
-      cr.z[d + 1] = (uint32_t)(w * p.nxy) * SB;
+      zr[d + 1] = (uint32_t)(w * p.nxy) * SB;
     }
+    return zc >= 0;
+  };
+
+  // pull the column at rows (xr, yr, zr); density (Dirichlet-overridden) to rho
+  auto pull_col = [&](const uint32_t (&xr)[3], const uint32_t (&yr)[3], const uint32_t (&zr)[3], R (&f)[C][Q],
+                      R (&rho)[C]) {
+    RowsB cr;
 #pragma unroll
-    for (int k = 0; k < NCP; k++) {
-      const int c = t + k * NT;
-      if (c >= NCOL) continue;
-      R rho[C];
+    for (int d = 0; d < 3; d++) {
+      cr.x[d] = xr[d];
+      cr.y[d] = yr[d];
+      cr.z[d] = zr[d];
+    }
+    const uint32_t node = (cr.x[1] + cr.y[1] + cr.z[1]) / SB;
+    const uint32_t mk = b.mask[node];
 #pragma unroll
-      for (int m = 0; m < C; m++) rho[m] = (R)NAN;
-      if (cval[k] && zc >= 0) {
+    for (int m = 0; m < C; m++) rho[m] = (R)NAN;
+    if (mk & kSelfSolid) return;
+    pull_buf<C, S, R>(p, rin, mk, cr, plane_b, f);
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-          cr.x[d] = cx[k][d];
-          cr.y[d] = cy[k][d];
-        }
-        const uint32_t node = (cr.x[1] + cr.y[1] + cr.z[1]) / SB;
-        const uint32_t mk = b.mask[node];
-        if (!(mk & kSelfSolid)) {
-          R f[C][Q];
-          pull_buf<C, S, R>(p, rin, mk, cr, plane_b, f);
+    for (int m = 0; m < C; m++) {
+      R s = (R)0;
 #pragma unroll
-          for (int m = 0; m < C; m++) {
-            R s = (R)0;
+      for (int n = 0; n < Q; n++) s = s + f[m][n];
+      rho[m] = s;
+    }
+    if (mk & (0x3fu << kFaceShift)) {
 #pragma unroll
-            for (int n = 0; n < Q; n++) s = s + f[m][n];
-            rho[m] = s;
-          }
-          if (mk & (0x3fu << kFaceShift)) {
+      for (int fc = 0; fc < 6; fc++)
+        if (mk & (1u << (kFaceShift + fc)))
 #pragma unroll
-            for (int fc = 0; fc < 6; fc++)
-              if (mk & (1u << (kFaceShift + fc)))
-#pragma unroll
-                for (int m = 0; m < C; m++) rho[m] = p.bcrho[fc][m];
-          }
-        }
-      }
-#pragma unroll
-      for (int m = 0; m < C; m++) srho[(slot * C + m) * NCOL + c] = rho[m];
+          for (int m = 0; m < C; m++) rho[m] = p.bcrho[fc][m];
     }
   };
 
   auto slot_of = [&](int zz) { return (zz - (z0 - RR) + NPL * 4096) % NPL; };
 
-  if constexpr (RR > 0) {
-    for (int zz = z0 - RR; zz < z0 + RR; zz++) phase_a(zz, slot_of(zz));
-  }
-  for (int z = z0; z < z1; z++) {
-    if constexpr (RR > 0) {
-      phase_a(z + RR, slot_of(z + RR));
-      __syncthreads();
-    }
-    if (active) {
-      set_z<3, 1, R>(p, z, nb);
+  // phase A for plane zz: own column into f (kept for phase B), ring columns
+  auto phase_a = [&](int zz, R (&f)[C][Q]) {
+    const int slot = slot_of(zz);
+    uint32_t zr[3];
+    const bool zval = zrows(zz, zr);
+    R rho[C];
 #pragma unroll
-      for (int d = 0; d < 3; d++) rb.z[d] = (uint32_t)nb.zw[d] * SB;
+    for (int m = 0; m < C; m++) rho[m] = (R)NAN;
+    if (oval && zval) pull_col(ox, oy, zr, f, rho);
+#pragma unroll
+    for (int m = 0; m < C; m++) srho[(slot * C + m) * NCOL + ocol] = rho[m];
+#pragma unroll
+    for (int k = 0; k < NRP; k++) {
+      if (rcol[k] < 0) continue;
+      R rr[C];
+#pragma unroll
+      for (int m = 0; m < C; m++) rr[m] = (R)NAN;
+      if (rval[k] && zval) {
+        R fr[C][Q];
+        pull_col(rx[k], ry[k], zr, fr, rr);
+      }
+#pragma unroll
+      for (int m = 0; m < C; m++) srho[(slot * C + m) * NCOL + rcol[k]] = rr[m];
+    }
+  };
+
+  // phase B for plane z with the own column's populations f
+  auto phase_b = [&](int z, R (&f)[C][Q]) {
+    if (!active) return;
+    set_z<3, 1, R>(p, z, nb);
+    const int node = nb.xw[1] + nb.yw[1] + nb.zw[1];
+    const uint32_t mk = b.mask[node];
+    if (mk & kSelfSolid) return;
+    if (mk & (0x3fu << kFaceShift)) apply_dirichlet<3, C, R>(p, mk, f);
+    LdsRho<R, C, RR, EXW, EYW> acc;
+    acc.s = srho;
+    acc.col = ocol;
+#pragma unroll
+    for (int dz = -RR; dz <= RR; dz++) acc.slot[dz + RR] = slot_of(z + dz);
+    R rs[C];
+#pragma unroll
+    for (int m = 0; m < C; m++) rs[m] = acc.template get<0, 0, 0>(m);
+    R rl[C], j[C][3], F[C][3], up[3];
+    moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
+    collide_store<3, C, FAST, R>(p, BufStore<S, R>{p, rout, (uint32_t)node * SB, plane_b}, f, rl, F, up);
+  };
+
+  if constexpr (RR == 0) {
+    // no stencil: density from the node's own populations
+    for (int z = z0; z < z1; z++) {
+      if (!active) break;
+      set_z<3, 1, R>(p, z, nb);
       const int node = nb.xw[1] + nb.yw[1] + nb.zw[1];
       const uint32_t mk = b.mask[node];
-      if (!(mk & kSelfSolid)) {
-        R f[C][Q];
-        pull_buf<C, S, R>(p, rin, mk, rb, plane_b, f);
-        R rs[C];
-        if constexpr (RR == 0) {
+      if (mk & kSelfSolid) continue;
+      RowsB cr;
 #pragma unroll
-          for (int m = 0; m < C; m++) {
-            R s = (R)0;
-#pragma unroll
-            for (int n = 0; n < Q; n++) s = s + f[m][n];
-            rs[m] = s;
-          }
-          if (mk & (0x3fu << kFaceShift)) {
-#pragma unroll
-            for (int fc = 0; fc < 6; fc++)
-              if (mk & (1u << (kFaceShift + fc)))
-#pragma unroll
-                for (int m = 0; m < C; m++) rs[m] = p.bcrho[fc][m];
-          }
-        }
-        if (mk & (0x3fu << kFaceShift)) apply_dirichlet<3, C, R>(p, mk, f);
-        LdsRho<R, C, RR, EXW, EYW> acc;
-        acc.s = srho;
-        acc.col = (ty + RR) * EXW + tx + RR;
-#pragma unroll
-        for (int dz = -RR; dz <= RR; dz++) acc.slot[dz + RR] = slot_of(z + dz);
-        if constexpr (RR > 0) {
-#pragma unroll
-          for (int m = 0; m < C; m++) rs[m] = acc.template get<0, 0, 0>(m);
-        }
-        R rl[C], j[C][3], F[C][3], up[3];
-        moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
-        collide_store<3, C, FAST, R>(p, BufStore<S, R>{p, rout, (uint32_t)node * SB, plane_b}, f, rl, F, up);
+      for (int d = 0; d < 3; d++) {
+        cr.x[d] = ox[d];
+        cr.y[d] = oy[d];
+        cr.z[d] = (uint32_t)nb.zw[d] * SB;
       }
+      R f[C][Q];
+      pull_buf<C, S, R>(p, rin, mk, cr, plane_b, f);
+      R rs[C];
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+        R s2 = (R)0;
+#pragma unroll
+        for (int n = 0; n < Q; n++) s2 = s2 + f[m][n];
+        rs[m] = s2;
+      }
+      if (mk & (0x3fu << kFaceShift)) {
+#pragma unroll
+        for (int fc = 0; fc < 6; fc++)
+          if (mk & (1u << (kFaceShift + fc)))
+#pragma unroll
+            for (int m = 0; m < C; m++) rs[m] = p.bcrho[fc][m];
+        apply_dirichlet<3, C, R>(p, mk, f);
+      }
+      LdsRho<R, C, 0, EXW, EYW> acc;  // unused without a stencil
+      acc.s = srho;
+      acc.col = 0;
+      acc.slot[0] = 0;
+      R rl[C], j[C][3], F[C][3], up[3];
+      moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
+      collide_store<3, C, FAST, R>(p, BufStore<S, R>{p, rout, (uint32_t)node * SB, plane_b}, f, rl, F, up);
     }
-    if constexpr (RR > 0) __syncthreads();
+  } else if constexpr (!KEEP) {
+    // registers too few for a ring of RR+1 planes: phase B re-pulls plane z
+    // (read RR planes earlier by this workgroup, so mostly from L2)
+    R f[C][Q];
+    for (int zz = z0 - RR; zz < z0 + RR; zz++) phase_a(zz, f);
+    for (int z = z0; z < z1; z++) {
+      phase_a(z + RR, f);
+      __syncthreads();
+      if (active) {
+        uint32_t zr[3];
+        zrows(z, zr);
+        RowsB cr;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          cr.x[d] = ox[d];
+          cr.y[d] = oy[d];
+          cr.z[d] = zr[d];
+        }
+        const uint32_t mk = b.mask[(cr.x[1] + cr.y[1] + cr.z[1]) / SB];
+        if (!(mk & kSelfSolid)) pull_buf<C, S, R>(p, rin, mk, cr, plane_b, f);
+      }
+      phase_b(z, f);
+      __syncthreads();
+    }
+  } else if constexpr (RR == 1) {
+    // register ring of two planes: the populations of plane z+1 pulled in
+    // phase A are the ones phase B collides one plane later
+    R fa[C][Q], fb[C][Q];
+    phase_a(z0 - 1, fb);  // density only
+    phase_a(z0, fa);
+    auto step = [&](int z, R (&cur)[C][Q], R (&nxt)[C][Q]) {
+      phase_a(z + 1, nxt);
+      __syncthreads();
+      phase_b(z, cur);
+      __syncthreads();
+    };
+    for (int z = z0; z < z1; z += 2) {
+      step(z, fa, fb);
+      if (z + 1 < z1) step(z + 1, fb, fa);
+    }
+  } else {
+    // register ring of three planes
+    R f0[C][Q], f1[C][Q], f2[C][Q];
+    phase_a(z0 - 2, f2);  // density only
+    phase_a(z0 - 1, f2);  // density only
+    phase_a(z0, f0);
+    phase_a(z0 + 1, f1);
+    auto step = [&](int z, R (&cur)[C][Q], R (&nxt)[C][Q]) {
+      phase_a(z + 2, nxt);
+      __syncthreads();
+      phase_b(z, cur);
+      __syncthreads();
+    };
+    for (int z = z0; z < z1; z += 3) {
+      step(z, f0, f2);
+      if (z + 1 < z1) step(z + 1, f1, f0);
+      if (z + 2 < z1) step(z + 2, f2, f1);
+    }
+  }
+}
+
+template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(TXG_FUSED_WAVES_PER_EU)))
+k_fused3(Params<R> p, Bufs<S, R> b, FusedGeom g) {
+  __shared__ R srho[FusedShape<NFF, TX, TY>::template lds_len<C>()];
+  const int lb = xcd_remap(blockIdx.x, g.nblocks);
+  const int tile = lb % g.ntiles, seg = lb / g.ntiles;
+  const int z0 = seg * g.zseg;
+  fused3_item<C, NFF, FAST, S, R, TX, TY>(p, b, (tile % g.ntx) * TX, (tile / g.ntx) * TY, z0,
+                                          min(p.nz, z0 + g.zseg), srho);
+}
+
+// one member domain of a batched one-pass launch: its parameters, its two
+// buffer orientations (b[0] reads f[0] and writes f[1], b[1] the reverse),
+// its launch geometry and the orientation of its next step at flip = 0
+template <class S, class R>
+struct BatchMember {
+  Params<R> p;
+  Bufs<S, R> b[2];
+  FusedGeom g;
+  int parity;
+};
+
+// Batched one-pass step: many independent domains (any sizes) in one
+// launch.  The work items (tile x z-segment of every member) are numbered
+// member after member; prefix[d] is the first item of member d, prefix[nmem]
+// the total.  The grid is persistent: workgroup k of XCD x walks the items of
+// the x-th eighth of the list with stride (workgroups on that XCD), so an
+// XCD's L2 serves contiguous tiles of the same member.  flip toggles every
+// member's orientation (every step flips all of them).
+template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(TXG_FUSED_WAVES_PER_EU)))
+k_fused3_batch(const BatchMember<S, R>* __restrict__ mem, const int* __restrict__ prefix, int nmem, int flip) {
+  __shared__ R srho[FusedShape<NFF, TX, TY>::template lds_len<C>()];
+  constexpr int NX = 8;
+  const int nitems = prefix[nmem];
+  const int xcd = blockIdx.x % NX, k = blockIdx.x / NX;
+  const int nwg = ((int)gridDim.x - xcd + NX - 1) / NX;  // workgroups on this XCD
+  const int lo = (int)((long long)nitems * xcd / NX), hi = (int)((long long)nitems * (xcd + 1) / NX);
+  for (int it = lo + k; it < hi; it += nwg) {
+    int a = 0, e = nmem;  // prefix[a] <= it < prefix[e]
+    while (e - a > 1) {
+      const int mid = (a + e) >> 1;
+      if (prefix[mid] <= it) a = mid;
+      else e = mid;
+    }
+    const BatchMember<S, R>& m = mem[a];
+    const int lb = it - prefix[a];
+    const int tile = lb % m.g.ntiles, seg = lb / m.g.ntiles;
+    const int z0 = seg * m.g.zseg;
+    fused3_item<C, NFF, FAST, S, R, TX, TY>(m.p, m.b[m.parity ^ flip], (tile % m.g.ntx) * TX,
+                                            (tile / m.g.ntx) * TY, z0, min(m.p.nz, z0 + m.g.zseg), srho);
   }
 }
 

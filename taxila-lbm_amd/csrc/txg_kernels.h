@@ -75,6 +75,7 @@ struct Params {
   R kax, kdg;          // fluid-solid 1./6. (1./3.) and 1./12. as default reals
   R bcrho[6][MAXC];
   R shift0[MAXC];      // FP16 storage: stored = f - w_n * shift0[m]
+  R shift[MAXC][19];   // w_n * shift0[m], rounded once on the host
   // reciprocals used by the FAST (FP32) arithmetic
   R omega[MAXC];       // 1/tau
   R ic2[MAXC];         // 1/c_s2
@@ -99,12 +100,14 @@ __device__ __forceinline__ R ld(const S* p, int i, R shift) {
   else
     return (R)p[i];
 }
+// FP32/FP64 populations are stored non-temporally (the step does not read
+// them again); half-float stores keep the default cache policy
 template <class S, class R>
 __device__ __forceinline__ void st(S* p, int i, R v, R shift) {
   if constexpr (std::is_same<S, __half>::value)
     p[i] = __float2half_rn((float)(v - shift));
   else
-    p[i] = (S)v;
+    __builtin_nontemporal_store((S)v, p + i);
 }
 
 // compile-time loop
@@ -184,7 +187,7 @@ __device__ __forceinline__ void pull(const Params<R>& p, const S* __restrict__ f
     // w_opp(n) = w_n, so the storage shift does not depend on the branch
 #pragma unroll
     for (int m = 0; m < C; m++)
-      f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + dir) * p.Np, src, (R)L::w[n] * p.shift0[m]);
+      f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + dir) * p.Np, src, p.shift[m][n]);
   });
 }
 
@@ -196,7 +199,7 @@ __device__ __forceinline__ void load_local(const Params<R>& p, const S* __restri
 #pragma unroll
   for (int n = 0; n < L::Q; n++)
 #pragma unroll
-    for (int m = 0; m < C; m++) f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + n) * p.Np, node, (R)L::w[n] * p.shift0[m]);
+    for (int m = 0; m < C; m++) f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + n) * p.Np, node, p.shift[m][n]);
 }
 
 // ------------------------------------------------------------ Dirichlet BC
@@ -735,7 +738,7 @@ struct PtrStore {
   S* __restrict__ fout;
   int node;
   __device__ __forceinline__ void operator()(int m, int n, R v) const {
-    st<S, R>(fout + (size_t)(m * Lat<D>::Q + n) * p.Np, node, v, (R)Lat<D>::w[n] * p.shift0[m]);
+    st<S, R>(fout + (size_t)(m * Lat<D>::Q + n) * p.Np, node, v, p.shift[m][n]);
   }
 };
 
@@ -870,7 +873,7 @@ __global__ void __launch_bounds__(256) k_fiinit(Params<R> p, Bufs<S, R> b, const
         });
         feq = (R)L::w[n] * r * (p.a15[m] + ue / p.cs2[m] + ue * ue / p.two_cs2_sq[m] - usqr / p.two_cs2[m]);
       }
-      st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, ((R)1. - (R)0.5 * pref) * feq, (R)L::w[n] * p.shift0[m]);
+      st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, ((R)1. - (R)0.5 * pref) * feq, p.shift[m][n]);
     });
   }
 }

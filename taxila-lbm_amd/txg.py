@@ -27,6 +27,7 @@ SRT, MRT = 0, 1
 FP64, FP32, FP32_FP16 = 0, 1, 2
 BC_NULL, BC_DENSITY = 0, 6
 FACES = ("xm", "xp", "ym", "yp", "zm", "zp")
+OK, ARG_ERROR, HIP_ERROR, STATE_ERROR, UNSUPPORTED_ERROR, NOMEM_ERROR = 0, 1, 2, 3, 4, 5
 PETSC_VEC_CLASSID = 1211214  # PetscBinaryIO.py:113
 
 
@@ -158,7 +159,9 @@ _lib = None
 SYMBOLS = ("txg_desc_init", "txg_create", "txg_set_walls", "txg_set_state", "txg_set_fi", "txg_init",
            "txg_step", "txg_get_fi", "txg_get_moments", "txg_diagnostics", "txg_reduce",
            "txg_delta_norm", "txg_sync", "txg_last_error", "txg_destroy", "txg_profile_enable",
-           "txg_profile_read", "txg_tune")
+           "txg_profile_read", "txg_tune", "txg_tune_info", "txg_batch_create", "txg_batch_add",
+           "txg_batch_remove", "txg_batch_step", "txg_batch_size", "txg_batch_tune", "txg_batch_tune_info",
+           "txg_batch_sync", "txg_batch_last_error", "txg_batch_destroy")
 
 
 class TxgError(RuntimeError):
@@ -188,6 +191,17 @@ def lib():
         L.txg_profile_enable.argtypes = [vp, C.c_int32]
         L.txg_profile_read.argtypes = [vp, vp, vp]
         L.txg_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        L.txg_tune_info.argtypes = [vp, vp, vp, vp, vp]
+        L.txg_batch_create.argtypes = [C.POINTER(vp)]
+        L.txg_batch_add.argtypes = [vp, vp]
+        L.txg_batch_remove.argtypes = [vp, vp]
+        L.txg_batch_step.argtypes = [vp, C.c_int32]
+        L.txg_batch_size.argtypes = [vp, vp]
+        L.txg_batch_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+        L.txg_batch_tune_info.argtypes = [vp, vp, vp, vp, vp]
+        L.txg_batch_sync.argtypes = [vp]
+        L.txg_batch_last_error.argtypes = [vp, C.c_char_p, C.c_int32]
+        L.txg_batch_destroy.argtypes = [vp]
         for name in SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -306,3 +320,78 @@ class Domain:
         tile (32, 8) / (64, 4) and the z-planes per workgroup."""
         self._check(lib().txg_tune(self.h, self.KERNELS[kernel], int(tile[0]), int(tile[1]), int(zseg)),
                     "txg_tune")
+
+    def tune_info(self):
+        """(tile_x, tile_y, zseg, ms) of the one-pass launch shape in use, or
+        None before it is fixed (see txg_tune_info)."""
+        v = [C.c_int32(), C.c_int32(), C.c_int32()]
+        ms = C.c_float()
+        rc = lib().txg_tune_info(self.h, *[C.byref(x) for x in v], C.byref(ms))
+        if rc == STATE_ERROR:
+            return None
+        self._check(rc, "txg_tune_info")
+        return v[0].value, v[1].value, v[2].value, ms.value
+
+
+class Batch:
+    """Many independent D3Q19 domains stepped together (txg_batch_*).
+
+    Members are Domain objects; the batch holds references so they outlive
+    their membership."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        rc = lib().txg_batch_create(C.byref(h))
+        if rc:
+            raise TxgError("txg_batch_create failed (%d)" % rc)
+        self.h = h
+        self.members = []
+
+    def _check(self, rc, what):
+        if rc:
+            buf = C.create_string_buffer(512)
+            lib().txg_batch_last_error(self.h, buf, 512)
+            raise TxgError("%s failed (%d): %s" % (what, rc, buf.value.decode()))
+
+    def add(self, dom):
+        self._check(lib().txg_batch_add(self.h, dom.h), "txg_batch_add")
+        self.members.append(dom)
+
+    def remove(self, dom):
+        self._check(lib().txg_batch_remove(self.h, dom.h), "txg_batch_remove")
+        self.members.remove(dom)
+
+    def step(self, n=1):
+        self._check(lib().txg_batch_step(self.h, int(n)), "txg_batch_step")
+
+    def sync(self):
+        self._check(lib().txg_batch_sync(self.h), "txg_batch_sync")
+
+    def __len__(self):
+        n = C.c_int32()
+        self._check(lib().txg_batch_size(self.h, C.byref(n)), "txg_batch_size")
+        return n.value
+
+    def tune(self, tile=(0, 0), zseg=0):
+        self._check(lib().txg_batch_tune(self.h, int(tile[0]), int(tile[1]), int(zseg)), "txg_batch_tune")
+
+    def tune_info(self):
+        v = [C.c_int32(), C.c_int32(), C.c_int32()]
+        ms = C.c_float()
+        rc = lib().txg_batch_tune_info(self.h, *[C.byref(x) for x in v], C.byref(ms))
+        if rc == STATE_ERROR:
+            return None
+        self._check(rc, "txg_batch_tune_info")
+        return v[0].value, v[1].value, v[2].value, ms.value
+
+    def close(self):
+        if self.h:
+            lib().txg_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -134,3 +134,26 @@ def test_fp32_onepass_matches_twopass(prec):
     b = run_gpu(d, walls, rho, u, steps, "onepass").get_fi()
     tol = 1e-6 if prec == txg.FP32 else 2.0 ** -11 * 0.5
     assert np.abs(a - b).max() <= tol
+
+
+@pytest.mark.parametrize("prec", [txg.FP64, txg.FP32, txg.FP32_FP16])
+def test_autotuner_keeps_state_and_results(prec):
+    """The launch-shape autotuner times one-pass launches on the live state
+    at the first one-pass step.  The state must not advance while it does,
+    and the shape it picks must not change the arithmetic: an autotuned run
+    equals a run with a fixed shape bit for bit, in every precision."""
+    name, kw, wm, icm, steps = CASES[[c[0] for c in CASES].index("d3_porous_dirichlet")]
+    kw = dict(kw, precision=prec, rho0=[0.5, 0.5])
+    d, walls, rho, u = build_case(name, kw, wm, icm)
+    a = run_gpu(d, walls, rho, u, 0, "onepass")
+    assert a.tune_info() is None
+    a.step(steps)
+    tx, ty, zs, ms = a.tune_info()
+    assert (tx, ty) in ((32, 8), (64, 4)) and 1 <= zs <= d.nz and ms > 0
+    b = run_gpu(d, walls, rho, u, steps, "onepass", (32, 8), 3)
+    assert b.tune_info()[:3] == (32, 8, 3)
+    assert np.array_equal(a.get_fi(), b.get_fi())
+    if prec == txg.FP64:
+        o = cases.run(oracle.Oracle, d, walls, rho, u, steps).get_fi()
+        m = np.repeat(walls == 0, d.Q * d.ncomponents)
+        assert np.array_equal(a.get_fi()[m], o[m])
