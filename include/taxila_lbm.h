@@ -146,8 +146,20 @@ int txg_get_moments(txg_handle* h, double* rho, double* flux, double* forces);
 int txg_diagnostics(txg_handle* h, double* rhot, double* prs, double* velt);
 
 /* FP64 global sums over fluid nodes, reduced on the GPU:
- * out[m] = sum rho_m, out[C + m*D + d] = sum (j + F/2)_{m,d}. */
+ * out[m] = sum rho_m, out[C + m*D + d] = sum (j + F/2)_{m,d}.  These are
+ * the leading entries of txg_flow_sums. */
 int txg_reduce(txg_handle* h, double* out);
+
+/* All FP64 flow sums over fluid nodes (C + C*D + D + 1 values), computed
+ * on the GPU from the populations in place, in a fixed order (the result
+ * is deterministic): [0, C) sum rho_m; [C, C + C*D) sum (j + F/2)_{m,d}
+ * (the common-velocity momentum, lbm_flow.F90:518-524); [C + C*D, + D)
+ * sum of the output velocity u_d = sum_m mm_m (j + F/2)_{m,d} / rho_t
+ * (FlowUpdateDiagnosticsD3, lbm_flow.F90:654-708); last, the number of
+ * fluid nodes.  The mean velocity sum_u / (NX*NY*NZ) is the superficial
+ * (Darcy) velocity of a porous sample.  No reference counterpart as one
+ * call (the reference forms these with VecSum/VecNorm on its outputs). */
+int txg_flow_sums(txg_handle* h, double* out);
 
 /* DistributionCalcDeltaNorm on rho (lbm_distribution_function.F90:809-833):
  * max |(rho_old - rho)/rho| over owned node-components with rho != 0, then
@@ -217,6 +229,9 @@ int txg_batch_remove(txg_batch* b, txg_handle* h);
  * batched. */
 int txg_batch_step(txg_batch* b, int32_t nsteps);
 int txg_batch_size(const txg_batch* b, int32_t* n);
+/* txg_flow_sums of every member, in the order they were added:
+ * out[i*stride + k], stride >= C + C*D + D + 1.  Synchronises. */
+int txg_batch_flow_sums(txg_batch* b, double* out, int32_t stride);
 /* Launch shape of the batched kernel: tile (32x8 or 64x4) and z-planes per
  * work item, or 0,0,0 to let the autotuner time the candidates on the live
  * members at the next step (the default; the state is not advanced). */

@@ -25,40 +25,21 @@ namespace {
 
 thread_local std::string g_create_error;
 
-// reduction kernels (FP64, deterministic: fixed per-block order, then a
-// single-block pass in index order)
-__global__ void k_partial_sum(const double* __restrict__ a, const uint32_t* __restrict__ mask, int n,
-                              double* __restrict__ part) {
+// out[k] = sum of the nblocks partials of quantity k of k_sums (FP64, one
+// block per quantity, fixed tree order, so the sums are deterministic)
+__global__ void __launch_bounds__(256) k_sums_final(const double* __restrict__ part, int nblocks, int nv,
+                                                    double* __restrict__ out) {
   __shared__ double sh[256];
+  const int k = blockIdx.x;
   double s = 0.0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-    if (!(mask[i] & kSelfSolid)) s += a[i];
+  for (int i = threadIdx.x; i < nblocks; i += 256) s += part[(size_t)i * nv + k];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-__global__ void k_final_sum(const double* __restrict__ part, int nb, double* __restrict__ out) {
-  __shared__ double sh[256];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = sh[0];
-}
-
-// momentum of the common-velocity definition: j + F/2
-__global__ void k_j_half_f(double* __restrict__ out, const double* __restrict__ j, const double* __restrict__ F, int n) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) out[i] = j[i] + .5 * F[i];
+  if (threadIdx.x == 0) out[k] = sh[0];
 }
 
 // max |(old - new)/new| over node-components with new != 0; old := new
@@ -135,6 +116,7 @@ struct txg_handle {
   double* ex = nullptr;
   size_t ex_len = 0;
   double* red = nullptr;  // reduction scratch
+  double* sums = nullptr; // flow sums: kSumBlocks * nsums partials, then nsums results
   double* rho_old = nullptr;
   int have_old = 0;
   Params<double> p64{};
@@ -348,6 +330,21 @@ int run_export(txg_handle* h) {
   return h->hip(hipStreamSynchronize(h->stream), "export sync");
 }
 
+// queue the flow sums of the current state on the handle's stream; the
+// nsums results land in dst (device memory), default h->sums' tail
+int queue_flow_sums(txg_handle* h, double* dst) {
+  if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
+  int rc = ensure_rho(h);
+  if (rc) return rc;
+  AnyBufs b = bufs(h, h->cur);
+  const int nv = h->ks.nsums;
+  rc = h->hip(h->ks.sums(h->params(), b.ptr(), h->local_state ? 0 : 1, h->sums, h->stream), "flow sums kernel");
+  if (rc) return rc;
+  if (!dst) dst = h->sums + (size_t)kSumBlocks * nv;
+  hipLaunchKernelGGL(k_sums_final, dim3(nv), dim3(256), 0, h->stream, h->sums, kSumBlocks, nv, dst);
+  return h->hip(hipGetLastError(), "flow sums kernel");
+}
+
 }  // namespace
 
 extern "C" {
@@ -451,7 +448,8 @@ int txg_create(const txg_desc* d, txg_handle** out) {
                   hipMalloc((void**)&h->mask, (size_t)h->Np * 4) == hipSuccess &&
                   hipMalloc((void**)&h->tag, (size_t)h->Np) == hipSuccess &&
                   hipMalloc(&h->shift, (size_t)h->C * Q * R) == hipSuccess &&
-                  hipMalloc((void**)&h->red, 4096 * sizeof(double)) == hipSuccess;
+                  hipMalloc((void**)&h->red, 4096 * sizeof(double)) == hipSuccess &&
+                  hipMalloc((void**)&h->sums, (size_t)(kSumBlocks + 1) * h->ks.nsums * sizeof(double)) == hipSuccess;
   if (!alloc_ok) {
     txg_destroy(h);
     return bad(TXG_ERR_NOMEM, "device allocation failed");
@@ -968,29 +966,25 @@ int txg_diagnostics(txg_handle* h, double* rhot, double* prs, double* velt) {
 
 int txg_reduce(txg_handle* h, double* out) {
   if (!h || !out) return TXG_ERR_ARG;
-  int rc = run_export(h);
+  int rc = queue_flow_sums(h, nullptr);
   if (rc) return rc;
-  ExportPtrs e = export_ptrs(h);
-  const int nb = 1024;
-  const int C = h->C, D = h->D;
-  double* part = h->red;
-  double* res = h->red + nb;
-  // out[m] = sum rho_m ; out[C + m*D + d] = sum (j + F/2)_{m,d}
-  for (int m = 0; m < C; m++) {
-    hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(256), 0, h->stream, e.rho + (size_t)m * h->N, h->mask, h->N, part);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, h->stream, part, nb, res + m);
-  }
-  double* tmp = e.f;  // the population export is not needed here
-  for (int m = 0; m < C; m++)
-    for (int d = 0; d < D; d++) {
-      const size_t off = (size_t)(m * D + d) * h->N;
-      hipLaunchKernelGGL(k_j_half_f, dim3(cdiv(h->N, 256)), dim3(256), 0, h->stream, tmp, e.j + off, e.F + off, h->N);
-      hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(256), 0, h->stream, tmp, h->mask, h->N, part);
-      hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, h->stream, part, nb, res + C + m * D + d);
-    }
   rc = h->hip(hipStreamSynchronize(h->stream), "reduce sync");
   if (rc) return rc;
-  return h->hip(hipMemcpy(out, res, (size_t)(C + C * D) * 8, hipMemcpyDeviceToHost), "reduce download");
+  // mass and momentum are the leading C + C*D flow sums
+  return h->hip(hipMemcpy(out, h->sums + (size_t)kSumBlocks * h->ks.nsums, (size_t)(h->C + h->C * h->D) * 8,
+                          hipMemcpyDeviceToHost),
+                "reduce download");
+}
+
+int txg_flow_sums(txg_handle* h, double* out) {
+  if (!h || !out) return TXG_ERR_ARG;
+  int rc = queue_flow_sums(h, nullptr);
+  if (rc) return rc;
+  rc = h->hip(hipStreamSynchronize(h->stream), "flow sums sync");
+  if (rc) return rc;
+  return h->hip(hipMemcpy(out, h->sums + (size_t)kSumBlocks * h->ks.nsums, (size_t)h->ks.nsums * 8,
+                          hipMemcpyDeviceToHost),
+                "flow sums download");
 }
 
 int txg_delta_norm(txg_handle* h, double* norm) {
@@ -1039,6 +1033,7 @@ int txg_destroy(txg_handle* h) {
   (void)hipFree(h->shift);
   (void)hipFree(h->ex);
   (void)hipFree(h->red);
+  (void)hipFree(h->sums);
   (void)hipFree(h->rho_old);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1066,6 +1061,8 @@ struct txg_batch {
   size_t dmem_cap = 0;
   int* dprefix = nullptr;
   size_t dprefix_cap = 0;
+  double* dsums = nullptr;  // flow sums of every member
+  size_t dsums_len = 0;
   std::vector<txg_handle*> table;  // members in the device table, in order
   int table_dirty = 1;
   int flip = 0;  // batched steps since the table was written, mod 2
@@ -1348,6 +1345,38 @@ int txg_batch_step(txg_batch* b, int32_t nsteps) {
   return batch_release(b);
 }
 
+int txg_batch_flow_sums(txg_batch* b, double* out, int32_t stride) {
+  if (!b || !out) return TXG_ERR_ARG;
+  if (b->mem.empty()) return TXG_OK;
+  const int nv = b->ks.nsums;
+  if (stride < nv) return b->fail(TXG_ERR_ARG, "stride %d < %d flow sums per member", stride, nv);
+  const size_t n = b->mem.size();
+  if (n * nv > b->dsums_len) {
+    (void)hipFree(b->dsums);
+    b->dsums = nullptr;
+    b->dsums_len = 0;
+    if (hipMalloc(&b->dsums, n * nv * sizeof(double)) != hipSuccess)
+      return b->fail(TXG_ERR_NOMEM, "batch sums allocation");
+    b->dsums_len = n * nv;
+  }
+  // each member's sums on its own stream (they overlap), then gather
+  for (size_t i = 0; i < n; i++) {
+    txg_handle* h = b->mem[i];
+    int rc = queue_flow_sums(h, b->dsums + i * nv);
+    if (rc) return b->fail(rc, "member flow sums: %s", h->err.c_str());
+  }
+  int rc = batch_acquire(b);
+  if (rc) return rc;
+  std::vector<double> host(n * nv);
+  rc = b->hip(hipMemcpyAsync(host.data(), b->dsums, host.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream),
+              "batch sums download");
+  if (!rc) rc = b->hip(hipStreamSynchronize(b->stream), "batch sums sync");
+  if (rc) return rc;
+  for (size_t i = 0; i < n; i++)
+    for (int k = 0; k < nv; k++) out[i * (size_t)stride + k] = host[i * nv + k];
+  return TXG_OK;
+}
+
 int txg_batch_size(const txg_batch* b, int32_t* n) {
   if (!b || !n) return TXG_ERR_ARG;
   *n = (int32_t)b->mem.size();
@@ -1404,6 +1433,7 @@ int txg_batch_destroy(txg_batch* b) {
   for (txg_handle* h : b->mem) h->batch = nullptr;
   (void)hipFree(b->dmem);
   (void)hipFree(b->dprefix);
+  (void)hipFree(b->dsums);
   (void)hipEventDestroy(b->done);
   (void)hipStreamDestroy(b->stream);
   delete b;

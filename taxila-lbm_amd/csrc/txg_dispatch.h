@@ -35,6 +35,9 @@ struct KernelSet {
                                        hipStream_t s);
   // workgroups of the batched kernel resident per CU
   int (*fused_batch_occupancy[kNumTiles])();
+  // FP64 partial flow sums (k_sums): part[kSumBlocks * nsums]
+  hipError_t (*sums)(const void* p, const void* b, int pulled, double* part, hipStream_t s);
+  int nsums;
   size_t member_bytes;  // sizeof(BatchMember<S,R>)
 };
 

@@ -803,6 +803,81 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
   collide_store<D, C, FAST, R>(p, PtrStore<D, S, R>{p, b.fout, node}, f, rl, F, up);
 }
 
+// ------------------------------------------------------------ flow sums
+// Global FP64 sums of the flow state, computed from the populations in
+// place (no export): k_sums runs kSumBlocks blocks over the nodes in a fixed
+// grid-stride order, each writing NV partial sums, and k_sums_final adds the
+// partials of each quantity in a fixed tree order, so the result is
+// deterministic.  Quantities (fluid nodes only, as DistributionCalcDensity
+// zeroes walls): [0, C) rho_m; [C, C + C*D) (j + F/2)_{m,d}, the momentum of
+// the common-velocity definition (lbm_flow.F90:518-524); [C + C*D, + D) the
+// output velocity u_d = sum_m mm_m (j + F/2)_{m,d} / sum_m mm_m rho_m
+// (FlowUpdateDiagnosticsD3, lbm_flow.F90:654-708); last, the fluid count.
+constexpr int kSumBlocks = 1024;
+template <int D, int C>
+constexpr int num_sums() {
+  return C + C * D + D + 1;
+}
+
+template <int D, int C, int NFF, bool FAST, class S, class R>
+__global__ void __launch_bounds__(256) k_sums(Params<R> p, Bufs<S, R> b, int pulled, double* __restrict__ part) {
+  using L = Lat<D>;
+  constexpr int NV = num_sums<D, C>();
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = 0.0;
+  for (int node = blockIdx.x * 256 + threadIdx.x; node < p.N; node += gridDim.x * 256) {
+    const uint32_t mk = b.mask[node];
+    if (mk & kSelfSolid) continue;
+    int x, y, z;
+    node_xyz(p, node, x, y, z);
+    Nbr nb;
+    make_nbr<D>(p, x, y, z, nb);
+    R f[C][L::Q];
+    if (pulled)
+      pull<D, C, S, R>(p, b.fin, node, mk, nb, f);
+    else
+      load_local<D, C, S, R>(p, b.fin, node, f);
+    if (pulled && (mk & (0x3fu << kFaceShift))) apply_dirichlet<D, C, R>(p, mk, f);
+    R rs[C];
+#pragma unroll
+    for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
+    GlobalRho<R, 3> acc{b.rho, p.Np, nb};
+    R rl[C], j[C][3], F[C][3], up[3];
+    moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
+    double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      v[m] += (double)rl[m];
+      rt += (double)p.mm[m] * (double)rl[m];
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const double q = (double)j[m][d] + 0.5 * (double)F[m][d];
+        v[C + m * D + d] += q;
+        mom[d] += (double)p.mm[m] * q;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) v[C + C * D + d] += mom[d] / rt;
+    v[NV - 1] += 1.0;
+  }
+  // wave sums (fixed shuffle order), then the four waves in order
+  __shared__ double sm[4][NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) sm[wave][k] = s;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < NV) {
+    const int k = threadIdx.x;
+    part[(size_t)blockIdx.x * NV + k] = ((sm[0][k] + sm[1][k]) + sm[2][k]) + sm[3][k];
+  }
+}
+
 // FlowFiInit (lbm_flow.F90:923-934): f = (1 - pref/2) feq(rho_IC, u_IC) with
 // forces from the IC density; rho[] holds the IC density (NaN on solids).
 template <int D, int C, int NFF, class S, class R>

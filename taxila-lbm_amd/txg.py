@@ -161,7 +161,8 @@ SYMBOLS = ("txg_desc_init", "txg_create", "txg_set_walls", "txg_set_state", "txg
            "txg_delta_norm", "txg_sync", "txg_last_error", "txg_destroy", "txg_profile_enable",
            "txg_profile_read", "txg_tune", "txg_tune_info", "txg_batch_create", "txg_batch_add",
            "txg_batch_remove", "txg_batch_step", "txg_batch_size", "txg_batch_tune", "txg_batch_tune_info",
-           "txg_batch_sync", "txg_batch_last_error", "txg_batch_destroy")
+           "txg_batch_sync", "txg_batch_last_error", "txg_batch_destroy", "txg_flow_sums",
+           "txg_batch_flow_sums")
 
 
 class TxgError(RuntimeError):
@@ -192,6 +193,8 @@ def lib():
         L.txg_profile_read.argtypes = [vp, vp, vp]
         L.txg_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.txg_tune_info.argtypes = [vp, vp, vp, vp, vp]
+        L.txg_flow_sums.argtypes = [vp, vp]
+        L.txg_batch_flow_sums.argtypes = [vp, vp, C.c_int32]
         L.txg_batch_create.argtypes = [C.POINTER(vp)]
         L.txg_batch_add.argtypes = [vp, vp]
         L.txg_batch_remove.argtypes = [vp, vp]
@@ -297,6 +300,13 @@ class Domain:
         self._check(lib().txg_reduce(self.h, _p(out)), "txg_reduce")
         return out
 
+    def flow_sums(self):
+        """dict of the FP64 flow sums (txg_flow_sums): mass[C], momentum[C, D],
+        velocity_sum[D], fluid_nodes."""
+        out = np.empty(self.C + self.C * self.D + self.D + 1)
+        self._check(lib().txg_flow_sums(self.h, _p(out)), "txg_flow_sums")
+        return unpack_sums(out, self.C, self.D)
+
     def delta_norm(self):
         out = np.zeros(1)
         self._check(lib().txg_delta_norm(self.h, _p(out)), "txg_delta_norm")
@@ -333,6 +343,11 @@ class Domain:
         return v[0].value, v[1].value, v[2].value, ms.value
 
 
+def unpack_sums(v, C, D):
+    return {"mass": v[:C].copy(), "momentum": v[C:C + C * D].reshape(C, D).copy(),
+            "velocity_sum": v[C + C * D:C + C * D + D].copy(), "fluid_nodes": int(round(v[-1]))}
+
+
 class Batch:
     """Many independent D3Q19 domains stepped together (txg_batch_*).
 
@@ -366,6 +381,16 @@ class Batch:
 
     def sync(self):
         self._check(lib().txg_batch_sync(self.h), "txg_batch_sync")
+
+    def flow_sums(self):
+        """[txg_flow_sums dict] of every member, in membership order."""
+        if not self.members:
+            return []
+        c, d = self.members[0].C, self.members[0].D
+        nv = c + c * d + d + 1
+        out = np.empty(len(self.members) * nv)
+        self._check(lib().txg_batch_flow_sums(self.h, _p(out), nv), "txg_batch_flow_sums")
+        return [unpack_sums(out[i * nv:(i + 1) * nv], c, d) for i in range(len(self.members))]
 
     def __len__(self):
         n = C.c_int32()

@@ -10,6 +10,19 @@ import numpy as np
 
 import txg
 
+# lattice velocities and weights in the reference's direction order
+# (lbm_discretization_d3q19.F90:20-38, lbm_discretization_d2q9.F90:20-29)
+C3 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0],
+               [-1, 1, 0], [-1, -1, 0], [1, -1, 0], [1, 0, 1], [-1, 0, 1], [-1, 0, -1], [1, 0, -1], [0, 1, 1],
+               [0, -1, 1], [0, -1, -1], [0, 1, -1]])
+W3 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+C2 = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1]])
+W2 = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def lattice_c(ndims):
+    return C3 if ndims == 3 else C2
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 

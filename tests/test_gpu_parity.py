@@ -157,3 +157,39 @@ def test_autotuner_keeps_state_and_results(prec):
         o = cases.run(oracle.Oracle, d, walls, rho, u, steps).get_fi()
         m = np.repeat(walls == 0, d.Q * d.ncomponents)
         assert np.array_equal(a.get_fi()[m], o[m])
+
+
+def test_flow_sums_match_oracle():
+    """txg_flow_sums (FP64 sums on the GPU from the populations in place)
+    against the same sums formed in numpy from the oracle's outputs: mass,
+    momentum j + F/2 from its populations and forces, and the output
+    velocity from its diagnostics.  Summation orders differ, hence rtol
+    1e-12 (scaled by the sum of magnitudes)."""
+    name, kw, wm, icm, steps = CASES[[c[0] for c in CASES].index("d3_porous_dirichlet")]
+    d, walls, rho, u = build_case(name, kw, wm, icm)
+    g = run_gpu(d, walls, rho, u, steps)
+    s = g.flow_sums()
+    o = cases.run(oracle.Oracle, d, walls, rho, u, steps)
+    C, D, Q = d.ncomponents, d.ndims, d.Q
+    fl = walls == 0
+    ro, _, Fo = o.get_moments()
+    fo = o.get_fi().reshape(-1, Q, C)
+    j = np.einsum("nqm,qd->nmd", fo, cases.lattice_c(D).astype(float))
+    q = j + 0.5 * Fo.reshape(-1, D, C).transpose(0, 2, 1)
+    _, _, vo = o.diagnostics()
+
+    def close(a, b, mag):
+        assert abs(a - b) <= 1e-12 * mag, (a, b)
+
+    for m in range(C):
+        r = ro.reshape(-1, C)[fl, m]
+        close(s["mass"][m], r.sum(), np.abs(r).sum())
+        for dd in range(D):
+            close(s["momentum"][m, dd], q[fl, m, dd].sum(), np.abs(q[fl, m, dd]).sum())
+    v = vo.reshape(-1, D)[fl]
+    for dd in range(D):
+        close(s["velocity_sum"][dd], v[:, dd].sum(), np.abs(v[:, dd]).sum())
+    assert s["fluid_nodes"] == int(fl.sum())
+    # txg_reduce returns the leading mass and momentum sums
+    r = g.reduce()
+    assert np.array_equal(r[:C], s["mass"]) and np.array_equal(r[C:], s["momentum"].reshape(-1))

@@ -25,12 +25,7 @@ import cases
 import oracle
 import txg
 
-C3 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0],
-               [-1, 1, 0], [-1, -1, 0], [1, -1, 0], [1, 0, 1], [-1, 0, 1], [-1, 0, -1], [1, 0, -1], [0, 1, 1],
-               [0, -1, 1], [0, -1, -1], [0, 1, -1]])
-W3 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
-C2 = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1]])
-W2 = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+from cases import C2, C3, W2, W3  # noqa: E402
 
 
 def m_d3q19():
@@ -286,3 +281,17 @@ def test_dirichlet_faces_hold_density(dims):
     for face, xv in (("xm", 0), ("xp", nx - 1)):
         sel = fl & (x == xv)
         assert np.allclose(r[sel], np.array(bcv[face])[None, :], rtol=1e-13, atol=0)
+
+
+def test_permeability_samples_are_seeded():
+    """The porous samples of the batched permeability workload are a pure
+    function of their index: edge in 32..96, porosity drawn in [0.3, 0.5]
+    and reached to within one sphere's volume."""
+    import permeability as pm
+    for i in (0, 5):
+        e1, p1, w1 = pm.sample(i)
+        e2, p2, w2 = pm.sample(i)
+        assert e1 == e2 and p1 == p2 and np.array_equal(w1, w2)
+        assert 32 <= e1 <= 96 and 0.3 <= p1 <= 0.5
+        por = float((w1 == 0).mean())
+        assert p1 - 4.0 / 3.0 * np.pi * 10.0 ** 3 / e1 ** 3 <= por <= p1 + 1e-12
