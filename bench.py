@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--workload", default="c1_bubble128_fp32_bgk", choices=sorted(workloads.WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="length of the CPU baseline sample (0 disables it)")
+    ap.add_argument("--tile", default="", help="fix the one-pass tile, e.g. 64x4 (default: autotuned)")
+    ap.add_argument("--zseg", type=int, default=0, help="fix the z-planes per workgroup (default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -92,6 +94,9 @@ def main():
 
     d, walls, rho, u, descr = workloads.workload(args.workload)
     dom = txg.Domain(d)
+    if args.tile or args.zseg:
+        tile = tuple(int(v) for v in args.tile.split("x")) if args.tile else (0, 0)
+        dom.tune("auto", tile, args.zseg)
     dom.set_walls(walls)
     dom.set_state(rho, u)
     dom.init()

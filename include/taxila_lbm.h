@@ -195,6 +195,16 @@ int txg_tune(txg_handle* h, int32_t kernel, int32_t tile_x, int32_t tile_y, int3
  * fixed (no one-pass step yet and no explicit txg_tune). */
 int txg_tune_info(const txg_handle* h, int32_t* tile_x, int32_t* tile_y, int32_t* zseg, float* ms);
 
+/* The per-node link mask built by txg_set_walls (N entries, natural order):
+ * bit n (1..Q-1) set when the pull source x - c_n is a wall (a bounce-back
+ * link, DistributionBouncebackD3, lbm_distribution_function.F90:669-734);
+ * bit 19 when a neighbour x + c_n is a mineral (a wettability link,
+ * LBMAddFluidSolidForcesD3, lbm_forcing.F90:1326-1373); bit 20 when no wall
+ * lies within the psi-gradient stencil's reach; bits 24-29 when the node is
+ * on a Dirichlet face xm..zp; bit 31 alone on wall nodes.  Cells beyond a
+ * non-periodic face are walls (ghost tag 999, lbm_walls.F90:208-231). */
+int txg_get_mask(txg_handle* h, uint32_t* mask);
+
 /* Wait for all queued work on the handle's stream. */
 int txg_sync(txg_handle* h);
 

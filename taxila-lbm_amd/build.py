@@ -10,6 +10,7 @@ Outputs go to taxila-lbm_amd/build/ (git-ignored, shipped with the tree):
 from __future__ import annotations
 
 import concurrent.futures as cf
+import json
 import os
 import shutil
 import subprocess
@@ -41,13 +42,44 @@ UNITS = [
 ]
 
 
+RESOURCE_REMARK = "-Rpass-analysis=kernel-resource-usage"
+
+
 def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
-    out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
-                    if "packed-fp32-ops" not in l and l.strip())
+    lines = (r.stdout + r.stderr).splitlines()
     if r.returncode != 0:
+        out = "\n".join(l for l in lines if "packed-fp32-ops" not in l and l.strip())
         raise RuntimeError("command failed: %s\n%s" % (" ".join(cmd), out))
-    return out
+    res = _resources(lines)
+    out = "\n".join(l for l in lines if "packed-fp32-ops" not in l and "remark:" not in l and l.strip()
+                    and not l.lstrip().startswith(("|", "^", "In file included from"))
+                    and not l.strip()[0:1].isdigit())
+    return out, res
+
+
+def _resources(lines):
+    """{kernel symbol: {vgpr, agpr, sgpr, scratch, occupancy, lds}} from the
+    compiler's kernel-resource-usage remarks."""
+    keys = {"VGPRs": "vgpr", "AGPRs": "agpr", "SGPRs": "sgpr", "ScratchSize [bytes/lane]": "scratch",
+            "Occupancy [waves/SIMD]": "occupancy", "LDS Size [bytes/block]": "lds"}
+    res, cur = {}, None
+    for l in lines:
+        if "remark:" not in l:
+            continue
+        body = l.split("remark:", 1)[1].strip()
+        if body.startswith("Function Name:"):
+            cur = body.split(":", 1)[1].split("[")[0].strip()
+            res[cur] = {}
+        elif cur and ":" in body:
+            k, v = body.split(":", 1)
+            k = k.strip()
+            if k in keys:
+                try:
+                    res[cur][keys[k]] = int(v.split("[")[0].strip())
+                except ValueError:
+                    pass
+    return res
 
 
 def build_library(jobs: int = 4) -> str:
@@ -57,11 +89,16 @@ def build_library(jobs: int = 4) -> str:
     for src, extra in UNITS:
         obj = os.path.join(OUT, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + COMMON + extra + [RESOURCE_REMARK, "-c", os.path.join(CSRC, src), "-o", obj])
+    resources = {}
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for out in ex.map(_run, cmds):
+        for out, res in ex.map(_run, cmds):
             if out:
                 print(out, file=sys.stderr)
+            resources.update(res)
+    # per-kernel registers, scratch and occupancy, for the build checks in tests/
+    with open(os.path.join(OUT, "kernel_resources.json"), "w") as fh:
+        json.dump(resources, fh, indent=0, sort_keys=True)
     lib = os.path.join(OUT, "libtaxila_lbm.so")
     tmp = lib + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)

@@ -91,10 +91,88 @@ __global__ void k_load_fi(S* __restrict__ f, const double* __restrict__ src, int
 
 inline int cdiv(long a, int b) { return (int)((a + b - 1) / b); }
 
+// Link masks and mineral tags from the wall field (WallsSetGhostNodes +
+// WallsCommunicate, lbm_walls.F90:185-244: outside a non-periodic face the
+// cell is a 999 ghost, across a periodic face it is the periodic image).
+// Per pore node: bit n (1..Q-1) when the pull source x - c_n is a wall
+// (bounce-back link), kFSNear when some neighbour x + c_n is a mineral
+// (wettability link), kFFClean when no wall lies within the gradient
+// stencil's reach, bits kFaceShift+f on Dirichlet faces; solids get
+// kSelfSolid only.
+struct MaskGeom {
+  int nx, ny, nz, per[3], nminerals, reach, dirichlet, N;
+};
+
+template <int D>
+__device__ __forceinline__ double wall_cell(const double* __restrict__ w, const MaskGeom& g, int x, int y, int z) {
+  int c[3] = {x, y, z};
+  const int n3[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    if (c[k] < 0 || c[k] >= n3[k]) {
+      if (!g.per[k]) return 999.0;
+      c[k] = ((c[k] % n3[k]) + n3[k]) % n3[k];
+    }
+  }
+  return w[c[0] + (long)g.nx * (c[1] + (long)g.ny * c[2])];
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_build_mask(const double* __restrict__ w, MaskGeom g,
+                                                    uint32_t* __restrict__ mask, uint8_t* __restrict__ tag) {
+  using L = Lat<D>;
+  const int node = blockIdx.x * 256 + threadIdx.x;
+  if (node >= g.N) return;
+  const int x = node % g.nx, y = (node / g.nx) % g.ny, z = node / (g.nx * g.ny);
+  const double t = w[node];
+  const int mineral = (int)t;
+  tag[node] = (t > 0.0 && t < 998.0 && mineral >= 1 && mineral <= g.nminerals) ? (uint8_t)mineral : 0;
+  if (t != 0.0) {
+    mask[node] = kSelfSolid;
+    return;
+  }
+  uint32_t mk = 0;
+#pragma unroll
+  for (int n = 1; n < L::Q; n++) {
+    const int cx = L::cx[n], cy = L::cy[n], cz = D == 3 ? L::cz[n] : 0;
+    if (wall_cell<D>(w, g, x - cx, y - cy, z - cz) > 0.0) mk |= 1u << n;
+    const double tn = wall_cell<D>(w, g, x + cx, y + cy, z + cz);
+    const int mn = (int)tn;
+    if (tn > 0.0 && tn < 998.0 && mn >= 1 && mn <= g.nminerals) mk |= kFSNear;
+  }
+  bool clean = true;
+  const int rz = D == 3 ? g.reach : 0;
+  for (int dz = -rz; dz <= rz && clean; dz++)
+    for (int dy = -g.reach; dy <= g.reach && clean; dy++)
+      for (int dx = -g.reach; dx <= g.reach && clean; dx++)
+        if (wall_cell<D>(w, g, x + dx, y + dy, z + dz) != 0.0) clean = false;
+  if (clean) mk |= kFFClean;
+  const int pos[3] = {x, y, z}, hi[3] = {g.nx - 1, g.ny - 1, g.nz - 1};
+  for (int f = 0; f < 2 * D; f++) {
+    if (!(g.dirichlet & (1 << f))) continue;
+    const int dim = f / 2;
+    if (pos[dim] == ((f & 1) ? hi[dim] : 0)) mk |= 1u << (kFaceShift + f);
+  }
+  mask[node] = mk;
+}
+
 }  // namespace
 
 namespace {
 void batch_mark_dirty(txg_batch* b);
+
+// keep freed device memory in the default pool for reuse by later handles
+// (the pool would otherwise return it to the driver at synchronisations)
+void keep_pool_memory(int device) {
+  static bool done[64] = {};
+  if (device < 0 || device >= 64 || done[device]) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+    uint64_t thr = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+  }
+  done[device] = true;
+}
 }
 
 struct txg_handle {
@@ -122,8 +200,6 @@ struct txg_handle {
   Params<double> p64{};
   Params<float> p32{};
   std::vector<uint32_t> hmask;
-  std::vector<uint8_t> htag;
-  std::vector<int32_t> bb_node, bb_dir, fs_node, fs_dir, fs_min;
   // event timing of the step passes (txg_profile_enable)
   int prof = 0;
   int ev_next = 0;
@@ -155,6 +231,18 @@ struct txg_handle {
     return fail(TXG_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
   }
   const void* params() const { return prec == TXG_FP64 ? (const void*)&p64 : (const void*)&p32; }
+  // Device memory is stream-ordered (hipMallocAsync/hipFreeAsync on the
+  // handle's stream): freeing does not synchronise the device, so handles
+  // come and go while other handles' and batches' work keeps running.
+  template <class T>
+  bool alloc(T** p, size_t bytes) {
+    return hipMallocAsync((void**)p, bytes, stream) == hipSuccess;
+  }
+  template <class T>
+  void release(T*& p) {
+    if (p) (void)hipFreeAsync((void*)p, stream);
+    p = nullptr;
+  }
 };
 
 namespace {
@@ -247,18 +335,11 @@ void fill_params(const txg_desc& d, int N, int Np, int flags, int dirichlet, Par
   p.kdg = (R)(double)(1.f / 12.f);
 }
 
-const int kC3[19][3] = {{0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {-1, 0, 0}, {0, -1, 0}, {0, 0, 1},  {0, 0, -1},
-                        {1, 1, 0},  {-1, 1, 0}, {-1, -1, 0}, {1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {-1, 0, -1},
-                        {1, 0, -1}, {0, 1, 1},  {0, -1, 1}, {0, -1, -1}, {0, 1, -1}};
-const int kC2[9][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {-1, 0, 0}, {0, -1, 0},
-                       {1, 1, 0}, {-1, 1, 0}, {-1, -1, 0}, {1, -1, 0}};
-
 int ensure_export(txg_handle* h) {
   size_t need = (size_t)h->N * ((size_t)h->C * h->Q + h->C + h->D + 2 * (size_t)h->C * h->D);
   if (h->ex && h->ex_len >= need) return TXG_OK;
-  if (h->ex) (void)hipFree(h->ex);
-  h->ex = nullptr;
-  if (hipMalloc(&h->ex, need * sizeof(double)) != hipSuccess)
+  h->release(h->ex);
+  if (!h->alloc(&h->ex, need * sizeof(double)))
     return h->fail(TXG_ERR_NOMEM, "export buffer allocation failed (%zu doubles)", need);
   h->ex_len = need;
   return TXG_OK;
@@ -334,7 +415,8 @@ int run_export(txg_handle* h) {
 // nsums results land in dst (device memory), default h->sums' tail
 int queue_flow_sums(txg_handle* h, double* dst) {
   if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
-  int rc = ensure_rho(h);
+  // without a psi-gradient stencil k_sums forms each node's density itself
+  int rc = h->d.isotropy_order != 0 ? ensure_rho(h) : TXG_OK;
   if (rc) return rc;
   AnyBufs b = bufs(h, h->cur);
   const int nv = h->ks.nsums;
@@ -443,14 +525,13 @@ int txg_create(const txg_desc* d, txg_handle** out) {
   }
   const size_t R = h->prec == TXG_FP64 ? 8 : 4;
   const size_t fbytes = (size_t)h->C * Q * h->Np * h->ssize;
-  bool alloc_ok = hipMalloc(&h->f[0], fbytes) == hipSuccess && hipMalloc(&h->f[1], fbytes) == hipSuccess &&
-                  hipMalloc(&h->rho, (size_t)h->C * h->Np * R) == hipSuccess &&
-                  hipMalloc((void**)&h->mask, (size_t)h->Np * 4) == hipSuccess &&
-                  hipMalloc((void**)&h->tag, (size_t)h->Np) == hipSuccess &&
-                  hipMalloc(&h->shift, (size_t)h->C * Q * R) == hipSuccess &&
-                  hipMalloc((void**)&h->red, 4096 * sizeof(double)) == hipSuccess &&
-                  hipMalloc((void**)&h->sums, (size_t)(kSumBlocks + 1) * h->ks.nsums * sizeof(double)) == hipSuccess;
-  if (!alloc_ok) {
+  keep_pool_memory(h->device);
+  bool alloc_ok = h->alloc(&h->f[0], fbytes) && h->alloc(&h->f[1], fbytes) &&
+                  h->alloc(&h->rho, (size_t)h->C * h->Np * R) && h->alloc((void**)&h->mask, (size_t)h->Np * 4) &&
+                  h->alloc((void**)&h->tag, (size_t)h->Np) && h->alloc(&h->shift, (size_t)h->C * Q * R) &&
+                  h->alloc((void**)&h->red, 4096 * sizeof(double)) &&
+                  h->alloc((void**)&h->sums, (size_t)(kSumBlocks + 1) * h->ks.nsums * sizeof(double));
+  if (!alloc_ok || hipStreamSynchronize(h->stream) != hipSuccess) {
     txg_destroy(h);
     return bad(TXG_ERR_NOMEM, "device allocation failed");
   }
@@ -470,7 +551,7 @@ int txg_create(const txg_desc* d, txg_handle** out) {
       for (int m = 0; m < h->C; m++)
         for (int n = 0; n < Q; n++) {
           s[m * Q + n] = (float)wq[n] * h->p32.shift0[m];
-          h->p32.shift[m][n] = s[m * Q + n];
+          h->p32.shiftw[m][h->D == 3 ? wclass<3>(n) : wclass<2>(n)] = s[m * Q + n];
         }
       (void)hipMemcpy(h->shift, s.data(), s.size() * 4, hipMemcpyHostToDevice);
     }
@@ -486,8 +567,7 @@ int txg_create(const txg_desc* d, txg_handle** out) {
 int txg_set_walls(txg_handle* h, const double* walls) {
   if (!h || !walls) return TXG_ERR_ARG;
   const txg_desc& d = h->d;
-  const int nx = d.nx, ny = d.ny, nz = h->D == 3 ? d.nz : 1, N = h->N, Q = h->Q, D = h->D;
-  const int (*cv)[3] = D == 3 ? kC3 : kC2;
+  const int nx = d.nx, ny = d.ny, nz = h->D == 3 ? d.nz : 1, N = h->N, D = h->D;
   const int reach = d.isotropy_order == 8 ? 2 : (d.isotropy_order == 10 ? 3 : 1);
   for (int i = 0; i < N; i++) {
     double t = walls[i];
@@ -495,79 +575,32 @@ int txg_set_walls(txg_handle* h, const double* walls) {
     if (t == 900.0 || t == 901.0 || t == 902.0)
       return h->fail(TXG_ERR_UNSUPPORTED, "specular wall tags 900-902 are not implemented on the GPU");
   }
-  // tag value of the cell at (x,y,z), 999 outside a non-periodic face
-  // (WallsSetGhostNodes + WallsCommunicate, lbm_walls.F90:185-244)
-  auto cell = [&](int x, int y, int z, long* idx) -> double {
-    int c[3] = {x, y, z}, n3[3] = {nx, ny, nz};
-    for (int k = 0; k < D; k++) {
-      if (c[k] < 0 || c[k] >= n3[k]) {
-        if (!d.periodic[k]) return 999.0;
-        c[k] = ((c[k] % n3[k]) + n3[k]) % n3[k];
-      }
-    }
-    long i = c[0] + (long)nx * (c[1] + (long)ny * c[2]);
-    if (idx) *idx = i;
-    return walls[i];
-  };
-  h->hmask.assign(h->Np, 0u);
-  h->htag.assign(h->Np, 0);
-  h->bb_node.clear();
-  h->bb_dir.clear();
-  h->fs_node.clear();
-  h->fs_dir.clear();
-  h->fs_min.clear();
-  for (int i = 0; i < N; i++) {
-    double t = walls[i];
-    if (t != 0.0) {
-      int mineral = (int)t;
-      h->htag[i] = (t > 0.0 && t < 998.0 && mineral >= 1 && mineral <= d.nminerals) ? (uint8_t)mineral : 0;
-    }
-  }
-  for (int z = 0; z < nz; z++)
-    for (int y = 0; y < ny; y++)
-      for (int x = 0; x < nx; x++) {
-        long node = x + (long)nx * (y + (long)ny * z);
-        uint32_t mk = 0;
-        if (walls[node] != 0.0) {
-          h->hmask[node] = kSelfSolid;
-          continue;
-        }
-        for (int n = 1; n < Q; n++) {
-          // bounce-back link: the pull source x - c_n is a wall
-          double s = cell(x - cv[n][0], y - cv[n][1], z - cv[n][2], nullptr);
-          if (s > 0.0) {
-            mk |= 1u << n;
-            h->bb_node.push_back((int32_t)node);
-            h->bb_dir.push_back(n);
-          }
-          // wettability link: the neighbour x + c_n is a mineral wall
-          double tn = cell(x + cv[n][0], y + cv[n][1], z + cv[n][2], nullptr);
-          int mineral = (int)tn;
-          if (tn > 0.0 && tn < 998.0 && mineral >= 1 && mineral <= d.nminerals) {
-            mk |= kFSNear;
-            h->fs_node.push_back((int32_t)node);
-            h->fs_dir.push_back(n);
-            h->fs_min.push_back(mineral);
-          }
-        }
-        bool clean = true;
-        for (int dz = (D == 3 ? -reach : 0); dz <= (D == 3 ? reach : 0) && clean; dz++)
-          for (int dy = -reach; dy <= reach && clean; dy++)
-            for (int dx = -reach; dx <= reach && clean; dx++)
-              if (cell(x + dx, y + dy, z + dz, nullptr) != 0.0) clean = false;
-        if (clean) mk |= kFFClean;
-        const int pos[3] = {x, y, z}, hi[3] = {nx - 1, ny - 1, nz - 1};
-        for (int f = 0; f < 2 * D; f++) {
-          if (!(h->p64.dirichlet & (1 << f))) continue;
-          int dim = f / 2;
-          if (pos[dim] == ((f & 1) ? hi[dim] : 0)) mk |= 1u << (kFaceShift + f);
-        }
-        h->hmask[node] = mk;
-      }
-  int rc = h->hip(hipMemcpyAsync(h->mask, h->hmask.data(), (size_t)h->Np * 4, hipMemcpyHostToDevice, h->stream),
-                  "mask upload");
+  MaskGeom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  for (int k = 0; k < 3; k++) g.per[k] = k < D ? d.periodic[k] : 1;
+  g.nminerals = d.nminerals;
+  g.reach = reach;
+  g.dirichlet = h->p64.dirichlet;
+  g.N = N;
+  double* dw = nullptr;
+  int rc = h->hip(hipMallocAsync((void**)&dw, (size_t)N * 8, h->stream), "walls staging");
   if (rc) return rc;
-  rc = h->hip(hipMemcpyAsync(h->tag, h->htag.data(), (size_t)h->Np, hipMemcpyHostToDevice, h->stream), "tag upload");
+  rc = h->hip(hipMemcpyAsync(dw, walls, (size_t)N * 8, hipMemcpyHostToDevice, h->stream), "walls upload");
+  if (!rc) {
+    if (D == 3)
+      hipLaunchKernelGGL(k_build_mask<3>, dim3(cdiv(N, 256)), dim3(256), 0, h->stream, dw, g, h->mask, h->tag);
+    else
+      hipLaunchKernelGGL(k_build_mask<2>, dim3(cdiv(N, 256)), dim3(256), 0, h->stream, dw, g, h->mask, h->tag);
+    rc = h->hip(hipGetLastError(), "mask kernel");
+  }
+  (void)hipFreeAsync(dw, h->stream);
+  if (rc) return rc;
+  // the host keeps the masks for the solid test of the exports
+  h->hmask.assign(h->Np, 0u);
+  rc = h->hip(hipMemcpyAsync(h->hmask.data(), h->mask, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream),
+              "mask download");
   if (rc) return rc;
   rc = h->hip(hipStreamSynchronize(h->stream), "walls sync");
   if (rc) return rc;
@@ -582,8 +615,8 @@ int txg_set_state(txg_handle* h, const double* rho, const double* u) {
   const size_t R = h->prec == TXG_FP64 ? 8 : 4;
   double* tmp = nullptr;
   size_t nr = (size_t)h->N * h->C, nu = (size_t)h->N * h->C * h->D;
-  if (hipMalloc(&tmp, std::max(nr, nu) * 8) != hipSuccess) return h->fail(TXG_ERR_NOMEM, "staging allocation failed");
-  int rc = h->hip(hipMemcpy(tmp, rho, nr * 8, hipMemcpyHostToDevice), "rho upload");
+  if (!h->alloc(&tmp, std::max(nr, nu) * 8)) return h->fail(TXG_ERR_NOMEM, "staging allocation failed");
+  int rc = h->hip(hipMemcpyAsync(tmp, rho, nr * 8, hipMemcpyHostToDevice, h->stream), "rho upload");
   if (!rc) {
     if (R == 8)
       hipLaunchKernelGGL(k_fill_rho<double>, dim3(cdiv(h->N, 256)), dim3(256), 0, h->stream, (double*)h->rho, tmp,
@@ -594,20 +627,22 @@ int txg_set_state(txg_handle* h, const double* rho, const double* u) {
     rc = h->hip(hipStreamSynchronize(h->stream), "rho fill");
   }
   if (!rc) {
-    if (!h->u_ic && hipMalloc(&h->u_ic, nu * R) != hipSuccess) rc = h->fail(TXG_ERR_NOMEM, "u allocation failed");
+    if (!h->u_ic && !h->alloc(&h->u_ic, nu * R)) rc = h->fail(TXG_ERR_NOMEM, "u allocation failed");
   }
   if (!rc) {
     std::vector<double> uz(nu, 0.0);
     const double* us = u ? u : uz.data();
     if (R == 8) {
-      rc = h->hip(hipMemcpy(h->u_ic, us, nu * 8, hipMemcpyHostToDevice), "u upload");
+      rc = h->hip(hipMemcpyAsync(h->u_ic, us, nu * 8, hipMemcpyHostToDevice, h->stream), "u upload");
     } else {
       std::vector<float> uf(nu);
       for (size_t i = 0; i < nu; i++) uf[i] = (float)us[i];
-      rc = h->hip(hipMemcpy(h->u_ic, uf.data(), nu * 4, hipMemcpyHostToDevice), "u upload");
+      rc = h->hip(hipMemcpyAsync(h->u_ic, uf.data(), nu * 4, hipMemcpyHostToDevice, h->stream), "u upload");
     }
+    // the host vectors above must outlive the copies
+    if (!rc) rc = h->hip(hipStreamSynchronize(h->stream), "u upload sync");
   }
-  (void)hipFree(tmp);
+  h->release(tmp);
   if (rc) return rc;
   h->state_set = 1;
   h->fi_set = 0;
@@ -620,8 +655,8 @@ int txg_set_fi(txg_handle* h, const double* fi) {
   if (!h->walls_set) return h->fail(TXG_ERR_STATE, "txg_set_walls must come first");
   size_t n = (size_t)h->N * h->C * h->Q;
   double* tmp = nullptr;
-  if (hipMalloc(&tmp, n * 8) != hipSuccess) return h->fail(TXG_ERR_NOMEM, "staging allocation failed");
-  int rc = h->hip(hipMemcpy(tmp, fi, n * 8, hipMemcpyHostToDevice), "fi upload");
+  if (!h->alloc(&tmp, n * 8)) return h->fail(TXG_ERR_NOMEM, "staging allocation failed");
+  int rc = h->hip(hipMemcpyAsync(tmp, fi, n * 8, hipMemcpyHostToDevice, h->stream), "fi upload");
   if (!rc) {
     dim3 g(cdiv(h->N, 256));
     if (h->prec == TXG_FP64)
@@ -635,7 +670,7 @@ int txg_set_fi(txg_handle* h, const double* fi) {
                          h->C, h->Q, (const float*)h->shift);
     rc = h->hip(hipStreamSynchronize(h->stream), "fi load");
   }
-  (void)hipFree(tmp);
+  h->release(tmp);
   if (rc) return rc;
   // solid nodes keep NaN in the stencil density
   if (h->prec == TXG_FP64)
@@ -875,6 +910,13 @@ int txg_profile_read(txg_handle* h, double* ms, int32_t* n) {
   return TXG_OK;
 }
 
+int txg_get_mask(txg_handle* h, uint32_t* mask) {
+  if (!h || !mask) return TXG_ERR_ARG;
+  if (!h->walls_set) return h->fail(TXG_ERR_STATE, "txg_set_walls has not been called");
+  memcpy(mask, h->hmask.data(), (size_t)h->N * sizeof(uint32_t));
+  return TXG_OK;
+}
+
 int txg_sync(txg_handle* h) {
   if (!h) return TXG_ERR_ARG;
   return h->hip(hipStreamSynchronize(h->stream), "stream sync");
@@ -992,7 +1034,7 @@ int txg_delta_norm(txg_handle* h, double* norm) {
   int rc = run_export(h);
   if (rc) return rc;
   const size_t n = (size_t)h->N * h->C;
-  if (!h->rho_old && hipMalloc(&h->rho_old, n * 8) != hipSuccess) return h->fail(TXG_ERR_NOMEM, "rho_old allocation");
+  if (!h->rho_old && !h->alloc(&h->rho_old, n * 8)) return h->fail(TXG_ERR_NOMEM, "rho_old allocation");
   // export rho is [m*N + node]; the reference Vec is node-major, but max is
   // order independent
   ExportPtrs e = export_ptrs(h);
@@ -1024,17 +1066,20 @@ int txg_destroy(txg_handle* h) {
   if (h->batch) (void)txg_batch_remove(h->batch, h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->sync_ev) (void)hipEventDestroy(h->sync_ev);
-  (void)hipFree(h->f[0]);
-  (void)hipFree(h->f[1]);
-  (void)hipFree(h->rho);
-  (void)hipFree(h->mask);
-  (void)hipFree(h->tag);
-  (void)hipFree(h->u_ic);
-  (void)hipFree(h->shift);
-  (void)hipFree(h->ex);
-  (void)hipFree(h->red);
-  (void)hipFree(h->sums);
-  (void)hipFree(h->rho_old);
+  if (h->stream) {
+    h->release(h->f[0]);
+    h->release(h->f[1]);
+    h->release(h->rho);
+    h->release(h->mask);
+    h->release(h->tag);
+    h->release(h->u_ic);
+    h->release(h->shift);
+    h->release(h->ex);
+    h->release(h->red);
+    h->release(h->sums);
+    h->release(h->rho_old);
+    (void)hipStreamSynchronize(h->stream);
+  }
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

@@ -96,17 +96,25 @@ __device__ __forceinline__ void pull_buf(const Params<R>& p, const rsrc_t (&rin)
       off = (mk & (1u << n)) ? bo : so;
     }
 #pragma unroll
-    for (int m = 0; m < C; m++) f[m][n] = bld<S, R>(rin[m], off, p.shift[m][n]);
+    for (int m = 0; m < C; m++) f[m][n] = bld<S, R>(rin[m], off, storage_shift<3>(p, m, n));
   });
 }
 
 template <class S, class R>
 struct BufStore {
-  const Params<R>& p;
+  // holds copies, not a reference to the parameter block (see PtrStore)
   const rsrc_t* rout;
   uint32_t node_b, plane_b;
+  R sh[MAXC][3];
+  __device__ __forceinline__ BufStore(const Params<R>& p, const rsrc_t* r, uint32_t nb, uint32_t pb)
+      : rout(r), node_b(nb), plane_b(pb) {
+#pragma unroll
+    for (int m = 0; m < MAXC; m++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) sh[m][k] = p.shiftw[m][k];
+  }
   __device__ __forceinline__ void operator()(int m, int n, R v) const {
-    bst<S, R>(rout[m], node_b + (uint32_t)n * plane_b, v, p.shift[m][n]);
+    bst<S, R>(rout[m], node_b + (uint32_t)n * plane_b, v, sh[m][wclass<3>(n)]);
   }
 };
 
@@ -332,7 +340,7 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
     for (int m = 0; m < C; m++) rs[m] = acc.template get<0, 0, 0>(m);
     R rl[C], j[C][3], F[C][3], up[3];
     moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
-    collide_store<3, C, FAST, R>(p, BufStore<S, R>{p, rout, (uint32_t)node * SB, plane_b}, f, rl, F, up);
+    collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, rout, (uint32_t)node * SB, plane_b), f, rl, F, up);
   };
 
   if constexpr (RR == 0) {
@@ -374,7 +382,7 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
       acc.slot[0] = 0;
       R rl[C], j[C][3], F[C][3], up[3];
       moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
-      collide_store<3, C, FAST, R>(p, BufStore<S, R>{p, rout, (uint32_t)node * SB, plane_b}, f, rl, F, up);
+      collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, rout, (uint32_t)node * SB, plane_b), f, rl, F, up);
     }
   } else if constexpr (!KEEP) {
     // registers too few for a ring of RR+1 planes: phase B re-pulls plane z

@@ -75,7 +75,8 @@ struct Params {
   R kax, kdg;          // fluid-solid 1./6. (1./3.) and 1./12. as default reals
   R bcrho[6][MAXC];
   R shift0[MAXC];      // FP16 storage: stored = f - w_n * shift0[m]
-  R shift[MAXC][19];   // w_n * shift0[m], rounded once on the host
+  R shiftw[MAXC][3];   // w * shift0[m] for the three weight classes of the
+                       // lattice (rest, axis, diagonal), rounded on the host
   // reciprocals used by the FAST (FP32) arithmetic
   R omega[MAXC];       // 1/tau
   R ic2[MAXC];         // 1/c_s2
@@ -108,6 +109,19 @@ __device__ __forceinline__ void st(S* p, int i, R v, R shift) {
     p[i] = __float2half_rn((float)(v - shift));
   else
     __builtin_nontemporal_store((S)v, p + i);
+}
+
+// weight class of direction n: 0 rest, 1 axis, 2 diagonal (D3Q19 and D2Q9
+// order the directions rest, axes, diagonals)
+template <int D>
+__host__ __device__ constexpr int wclass(int n) {
+  return n == 0 ? 0 : (n <= 2 * D ? 1 : 2);
+}
+
+// FP16 storage shift w_n * shift0[m] of population (m, n)
+template <int D, class R>
+__device__ __forceinline__ R storage_shift(const Params<R>& p, int m, int n) {
+  return p.shiftw[m][wclass<D>(n)];
 }
 
 // compile-time loop
@@ -187,7 +201,7 @@ __device__ __forceinline__ void pull(const Params<R>& p, const S* __restrict__ f
     // w_opp(n) = w_n, so the storage shift does not depend on the branch
 #pragma unroll
     for (int m = 0; m < C; m++)
-      f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + dir) * p.Np, src, p.shift[m][n]);
+      f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + dir) * p.Np, src, storage_shift<D>(p, m, n));
   });
 }
 
@@ -199,7 +213,7 @@ __device__ __forceinline__ void load_local(const Params<R>& p, const S* __restri
 #pragma unroll
   for (int n = 0; n < L::Q; n++)
 #pragma unroll
-    for (int m = 0; m < C; m++) f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + n) * p.Np, node, p.shift[m][n]);
+    for (int m = 0; m < C; m++) f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + n) * p.Np, node, storage_shift<D>(p, m, n));
 }
 
 // ------------------------------------------------------------ Dirichlet BC
@@ -734,11 +748,20 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, const ST& stor
 // stores f*(node) into the SoA population array of the two-pass kernels
 template <int D, class S, class R>
 struct PtrStore {
-  const Params<R>& p;
+  // by value: a reference to the kernel's parameter block would make the
+  // compiler copy the whole block to scratch memory
   S* __restrict__ fout;
   int node;
+  int Np;
+  R sh[MAXC][3];
+  __device__ __forceinline__ PtrStore(const Params<R>& p, S* o, int nd) : fout(o), node(nd), Np(p.Np) {
+#pragma unroll
+    for (int m = 0; m < MAXC; m++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) sh[m][k] = p.shiftw[m][k];
+  }
   __device__ __forceinline__ void operator()(int m, int n, R v) const {
-    st<S, R>(fout + (size_t)(m * Lat<D>::Q + n) * p.Np, node, v, p.shift[m][n]);
+    st<S, R>(fout + (size_t)(m * Lat<D>::Q + n) * Np, node, v, sh[m][wclass<D>(n)]);
   }
 };
 
@@ -800,7 +823,7 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
     for (int d = 0; d < D; d++) ex.up[(size_t)d * p.N + node] = (double)up[d];
     return;
   }
-  collide_store<D, C, FAST, R>(p, PtrStore<D, S, R>{p, b.fout, node}, f, rl, F, up);
+  collide_store<D, C, FAST, R>(p, PtrStore<D, S, R>(p, b.fout, node), f, rl, F, up);
 }
 
 // ------------------------------------------------------------ flow sums
@@ -838,10 +861,29 @@ __global__ void __launch_bounds__(256) k_sums(Params<R> p, Bufs<S, R> b, int pul
       pull<D, C, S, R>(p, b.fin, node, mk, nb, f);
     else
       load_local<D, C, S, R>(p, b.fin, node, f);
-    if (pulled && (mk & (0x3fu << kFaceShift))) apply_dirichlet<D, C, R>(p, mk, f);
     R rs[C];
+    if constexpr (NFF == 0) {
+      // no stencil: the node's own density (Dirichlet faces prescribe it),
+      // as the density pass would store it
 #pragma unroll
-    for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
+      for (int m = 0; m < C; m++) {
+        R s = (R)0;
+#pragma unroll
+        for (int n = 0; n < L::Q; n++) s = s + f[m][n];
+        rs[m] = s;
+      }
+      if (pulled && (mk & (0x3fu << kFaceShift))) {
+#pragma unroll
+        for (int fc = 0; fc < 2 * D; fc++)
+          if (mk & (1u << (kFaceShift + fc)))
+#pragma unroll
+            for (int m = 0; m < C; m++) rs[m] = p.bcrho[fc][m];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
+    }
+    if (pulled && (mk & (0x3fu << kFaceShift))) apply_dirichlet<D, C, R>(p, mk, f);
     GlobalRho<R, 3> acc{b.rho, p.Np, nb};
     R rl[C], j[C][3], F[C][3], up[3];
     moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
@@ -948,7 +990,7 @@ __global__ void __launch_bounds__(256) k_fiinit(Params<R> p, Bufs<S, R> b, const
         });
         feq = (R)L::w[n] * r * (p.a15[m] + ue / p.cs2[m] + ue * ue / p.two_cs2_sq[m] - usqr / p.two_cs2[m]);
       }
-      st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, ((R)1. - (R)0.5 * pref) * feq, p.shift[m][n]);
+      st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, ((R)1. - (R)0.5 * pref) * feq, storage_shift<D>(p, m, n));
     });
   }
 }

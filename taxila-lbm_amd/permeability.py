@@ -29,6 +29,21 @@ import numpy as np
 import txg
 import workloads
 
+
+class Progress:
+    """A line on stderr at most every `every` seconds (long runs stay visibly alive)."""
+
+    def __init__(self, what, every=5.0):
+        self.what, self.every = what, every
+        self.t0 = self.last = time.perf_counter()
+
+    def __call__(self, msg):
+        now = time.perf_counter()
+        if now - self.last >= self.every:
+            self.last = now
+            print("[%s %.0fs] %s" % (self.what, now - self.t0, msg), file=sys.stderr, flush=True)
+
+
 G = 1e-6
 TAU = 1.0
 NU = (TAU - 0.5) / 3.0
@@ -88,6 +103,7 @@ def solve_batched(specs, check=100, precision=txg.FP32, tol=1e-7, max_steps=2000
     live = list(range(len(specs)))
     node_steps = 0
     t0 = time.perf_counter()
+    prog = Progress("batched")
     while live:
         b.step(check)
         node_steps += check * sum(specs[i][0] ** 3 for i in live)
@@ -99,6 +115,7 @@ def solve_batched(specs, check=100, precision=txg.FP32, tol=1e-7, max_steps=2000
             else:
                 keep.append(i)
         live = keep
+        prog("%d of %d samples running, step %d" % (len(live), len(specs), max(c.steps for c in conv)))
     sec = time.perf_counter() - t0
     b.close()
     out = [(c.permeability, c.steps) for c in conv]
@@ -111,6 +128,7 @@ def solve_plain(specs, check=100, precision=txg.FP32, tol=1e-7, max_steps=200000
     out = []
     node_steps = 0
     t0 = time.perf_counter()
+    prog = Progress("plain")
     for e, _, w in specs:
         dom = start(e, w, precision)
         c = Convergence(e ** 3, tol, max_steps)
@@ -121,17 +139,23 @@ def solve_plain(specs, check=100, precision=txg.FP32, tol=1e-7, max_steps=200000
                 break
         out.append((c.permeability, c.steps))
         dom.close()
+        prog("%d of %d samples done" % (len(out), len(specs)))
     return out, time.perf_counter() - t0, node_steps
 
 
 def main():
-    n = int(sys.argv[1]) if len(sys.argv) > 1 and not sys.argv[1].startswith("-") else 32
+    args = sys.argv[1:]
+    n = int(args[0]) if args and not args[0].startswith("-") else 32
+    tol = float(args[args.index("--tol") + 1]) if "--tol" in args else 1e-7
+    max_steps = int(args[args.index("--max-steps") + 1]) if "--max-steps" in args else 20000
+    prec = txg.FP64 if "--fp64" in args else txg.FP32
     specs = [sample(i) for i in range(n)]
-    res = {"samples": n, "nodes": int(sum(e ** 3 for e, _, _ in specs))}
-    kb, sb, nb = solve_batched(specs)
+    res = {"samples": n, "nodes": int(sum(e ** 3 for e, _, _ in specs)), "tol": tol, "max_steps": max_steps,
+           "precision": "fp64" if prec == txg.FP64 else "fp32"}
+    kb, sb, nb = solve_batched(specs, precision=prec, tol=tol, max_steps=max_steps)
     res["batched"] = {"seconds": sb, "MLUPS": nb / sb / 1e6, "steps": [s for _, s in kb]}
-    if "--plain" in sys.argv:
-        kp, sp, npl = solve_plain(specs)
+    if "--plain" in args:
+        kp, sp, npl = solve_plain(specs, precision=prec, tol=tol, max_steps=max_steps)
         res["plain"] = {"seconds": sp, "MLUPS": npl / sp / 1e6}
         res["speedup"] = sp / sb
         res["max_rel_diff_k"] = max(abs(a[0] - c[0]) / abs(c[0]) for a, c in zip(kb, kp) if c[0] != 0)

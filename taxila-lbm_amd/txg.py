@@ -162,7 +162,7 @@ SYMBOLS = ("txg_desc_init", "txg_create", "txg_set_walls", "txg_set_state", "txg
            "txg_profile_read", "txg_tune", "txg_tune_info", "txg_batch_create", "txg_batch_add",
            "txg_batch_remove", "txg_batch_step", "txg_batch_size", "txg_batch_tune", "txg_batch_tune_info",
            "txg_batch_sync", "txg_batch_last_error", "txg_batch_destroy", "txg_flow_sums",
-           "txg_batch_flow_sums")
+           "txg_batch_flow_sums", "txg_get_mask")
 
 
 class TxgError(RuntimeError):
@@ -194,6 +194,7 @@ def lib():
         L.txg_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.txg_tune_info.argtypes = [vp, vp, vp, vp, vp]
         L.txg_flow_sums.argtypes = [vp, vp]
+        L.txg_get_mask.argtypes = [vp, vp]
         L.txg_batch_flow_sums.argtypes = [vp, vp, C.c_int32]
         L.txg_batch_create.argtypes = [C.POINTER(vp)]
         L.txg_batch_add.argtypes = [vp, vp]
@@ -299,6 +300,12 @@ class Domain:
         out = np.empty(self.C + self.C * self.D)
         self._check(lib().txg_reduce(self.h, _p(out)), "txg_reduce")
         return out
+
+    def get_mask(self):
+        """uint32 link mask per node (txg_get_mask)."""
+        m = np.empty(self.N, dtype=np.uint32)
+        self._check(lib().txg_get_mask(self.h, _p(m)), "txg_get_mask")
+        return m
 
     def flow_sums(self):
         """dict of the FP64 flow sums (txg_flow_sums): mass[C], momentum[C, D],

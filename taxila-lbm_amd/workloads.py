@@ -71,7 +71,7 @@ def sphere_pack(shape, porosity, rmin, rmax, seed, minerals=(1.0, 2.0), periodic
 
 
 # --------------------------------------------------------------- configs
-def bubble_desc(ndims, n, precision=txg.FP32, relaxation_mode=txg.SRT, tau=1.0, rho0=0.5):
+def bubble_desc(ndims, n, precision=txg.FP32, relaxation_mode=txg.SRT, tau=1.0, rho0=RHO_MINOR):
     """Two-component Shan-Chen bubble: G = 1.8, tau_A = tau_B, periodic.
     MRT uses the standard D3Q19 rates s_e = 1.19, s_eps = 1.4, s_q = 1.2,
     s_nu = s_c = 1/tau, s_pi = s_m = 1 (reference defaults)."""
@@ -93,6 +93,58 @@ def bubble_ic(ndims, n, radius):
     return np.zeros(nn), rho, np.zeros(nn * 2 * ndims)
 
 
+# ---------------------------------------------------------------- drainage
+def wall_adhesion(theta_deg, g=G_BUBBLE, rho_major=RHO_MAJOR, rho_minor=RHO_MINOR):
+    """(gw_A, gw_B) giving contact angle theta (measured through component
+    A) by the relation of Huang, Thorne, Schaap & Sukop, Phys. Rev. E 76,
+    066701 (2007): cos(theta) = (G_ads,B - G_ads,A) / (G_c (rho_1 - rho_2)/2).
+    The reference's fluid-fluid force is -c_0 g rho_m sum ffw rho' e with
+    c_0 = 6 (lbm_forcing.F90:51-958) and its fluid-solid force
+    -gw rho_m sum k_n e with k_n = 1/6, 1/12 (lbm_forcing.F90:1326-1373), the
+    same weights as the isotropy-4 stencil, so G_c = 6 g and G_ads = gw.
+    With the isotropy-8 fluid-fluid stencil the relation is approximate."""
+    d = np.cos(np.radians(theta_deg)) * 6.0 * g * (rho_major - rho_minor) / 2.0
+    return -d / 2.0, d / 2.0
+
+
+DRAIN_TAU = (1.0, 0.6)
+DRAIN_DRHO = 0.01
+DRAIN_ANGLES = (45.0, 120.0)
+
+
+def drainage_desc(n, precision=txg.FP32, isotropy_order=8, shape=None):
+    """Two-phase drainage through an n^3 sphere pack: MRT (s_e = 1.19,
+    s_eps = 1.4, s_q = 1.2, s_nu = s_c = 1/tau), tau_A = 1.0, tau_B = 0.6,
+    g = 0.1, isotropy 8; component A (invading) enters at the x- face with
+    rho_A = 0.97 + 0.01 over the outlet (Dirichlet densities, BCApplyDirichlet,
+    lbm_bc.F90:1273-1333); the other faces are walls (ghost 999 bounce-back).
+    Mineral 1 wets A at 45 degrees, mineral 2 at 120 degrees."""
+    shape = shape or (n, n, n)
+    ta, tb = DRAIN_TAU
+    gw1 = wall_adhesion(DRAIN_ANGLES[0])
+    gw2 = wall_adhesion(DRAIN_ANGLES[1])
+    return txg.make_desc(ndims=3, ncomponents=2, shape=shape, periodic=(0, 0, 0), isotropy_order=isotropy_order,
+                         relaxation_mode=txg.MRT, precision=precision, tau=[ta, tb], mm=[1.0, 1.0],
+                         g=[[0.0, G_BUBBLE], [G_BUBBLE, 0.0]], s_c=[1 / ta, 1 / tb], s_nu=[1 / ta, 1 / tb],
+                         s_e=[1.19] * 2, s_e2=[1.4] * 2, s_q=[1.2] * 2, gw=[list(gw1), list(gw2)],
+                         bc={"xm": [RHO_MAJOR + DRAIN_DRHO, RHO_MINOR], "xp": [RHO_MINOR, RHO_MAJOR]},
+                         rho0=[RHO_MINOR, RHO_MINOR])
+
+
+def drainage_ic(n, seed, shape=None, inlet_layer=3):
+    """(walls, rho, u): porosity-0.40 pack of radii U[6, 12] (mineral per
+    sphere with probability 1/2, non-periodic), filled with B except an
+    inlet layer of A."""
+    shape = shape or (n, n, n)
+    walls = sphere_pack(shape, 0.40, 6.0, 12.0, seed, minerals=(1.0, 2.0), periodic=False)
+    nx = shape[0]
+    x = np.tile(np.arange(nx), int(np.prod(shape)) // nx)
+    a = np.where(x < inlet_layer, RHO_MAJOR, RHO_MINOR)
+    b = np.where(x < inlet_layer, RHO_MINOR, RHO_MAJOR)
+    rho = np.stack([a, b], axis=1).reshape(-1)
+    return walls, rho, np.zeros(int(np.prod(shape)) * 2 * 3)
+
+
 WORKLOADS = {
     # name: (description, desc builder, ic builder, nodes)
     "c1_bubble128_fp32_bgk": "D3Q19 two-component bubble 128^3, sphere r=24, periodic, G=1.8, "
@@ -102,6 +154,12 @@ WORKLOADS = {
     "c1_bubble128_fp64_bgk": "D3Q19 two-component bubble 128^3, BGK, FP64",
     "c1_bubble128_fp16_bgk": "D3Q19 two-component bubble 128^3, BGK, FP32 compute, shifted FP16 storage",
     "c0_bubble200_fp64": "D2Q9 two-component bubble 200^2, disc r=40, FP64 BGK",
+    "c2_drainage256_fp32": "D3Q19 two-phase drainage 256^3, sphere pack porosity 0.40 radii 6-12, two minerals "
+                           "(45/120 degrees), Dirichlet drho=0.01 on x, walls elsewhere, MRT isotropy 8, "
+                           "tau 1.0/0.6, FP32",
+    "c2_drainage256_fp16": "c2_drainage256 with FP32 compute and shifted-FP16 storage",
+    "c2_drainage256_fp64": "c2_drainage256 in FP64",
+    "c2_drainage64_fp32": "c2 recipe at 64^3, FP32",
 }
 
 
@@ -112,6 +170,12 @@ def workload(name):
         mode = txg.MRT if name.endswith("mrt") else txg.SRT
         d = bubble_desc(3, 128, precision=prec, relaxation_mode=mode)
         walls, rho, u = bubble_ic(3, 128, 24)
+        return d, walls, rho, u, WORKLOADS[name]
+    if name.startswith("c2_drainage"):
+        n = int(name.split("_")[1][len("drainage"):])
+        prec = {"fp32": txg.FP32, "fp64": txg.FP64, "fp16": txg.FP32_FP16}[name.split("_")[2]]
+        d = drainage_desc(n, precision=prec)
+        walls, rho, u = drainage_ic(n, seed=2024)
         return d, walls, rho, u, WORKLOADS[name]
     if name == "c0_bubble200_fp64":
         d = bubble_desc(2, 200, precision=txg.FP64)

@@ -77,3 +77,22 @@ def test_create_rejects_bad_descriptors_before_touching_the_device():
         buf = ctypes.create_string_buffer(256)
         txg.lib().txg_last_error(None, buf, 256)
         assert msg in buf.value.decode()
+
+
+def test_step_kernels_use_no_scratch():
+    """The one-pass and batched step kernels must keep everything in
+    registers and LDS: a kernel that touches scratch (private memory) runs
+    several times slower (a reference to the kernel's parameter block once
+    made the compiler copy that block to scratch in every lane).  The
+    build records each kernel's resources from the compiler's
+    kernel-resource-usage remarks."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(txg.LIB_PATH), "kernel_resources.json")
+    res = json.load(open(path))
+    fused = {k: v for k, v in res.items() if "k_fused3" in k}
+    assert len(fused) >= 2 * 2 * 3 * 6  # (single, batched) x tiles x precisions x configurations
+    bad = {k: v["scratch"] for k, v in fused.items() if v.get("scratch", 0) > 0 and "EddLi" not in k}
+    assert not bad, bad
+    for k, v in fused.items():
+        assert v.get("lds", 0) <= 64 * 1024, (k, v)

@@ -295,3 +295,20 @@ def test_permeability_samples_are_seeded():
         assert 32 <= e1 <= 96 and 0.3 <= p1 <= 0.5
         por = float((w1 == 0).mean())
         assert p1 - 4.0 / 3.0 * np.pi * 10.0 ** 3 / e1 ** 3 <= por <= p1 + 1e-12
+
+
+def test_request_stream_is_seeded_and_mixed():
+    """The serving workload's request stream: seeded, Poisson arrivals
+    (mean gap 1/rate), the 70/20/10 kind mix and the per-kind edge ranges."""
+    import serve
+    a = serve.make_stream(2000, rate=50.0, seed=3)
+    b = serve.make_stream(2000, rate=50.0, seed=3)
+    assert [(r.kind, r.seed, r.edge, r.budget, r.arrival) for r in a] == \
+           [(r.kind, r.seed, r.edge, r.budget, r.arrival) for r in b]
+    gaps = np.diff([0.0] + [r.arrival for r in a])
+    assert abs(gaps.mean() - 1 / 50.0) < 0.1 / 50.0
+    frac = {k: sum(r.kind == k for r in a) / len(a) for k in serve.KINDS}
+    assert abs(frac["perm"] - 0.7) < 0.04 and abs(frac["drain"] - 0.2) < 0.03 and abs(frac["bubble"] - 0.1) < 0.03
+    for r in a:
+        lo, hi = {"perm": (32, 128), "drain": (64, 160), "bubble": (256, 1024)}[r.kind]
+        assert lo <= r.edge <= hi
