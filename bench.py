@@ -40,13 +40,14 @@ def b_alg(desc):
 
 
 def traffic_for(name):
-    """HBM bytes per collide launch from the committed rocprofv3 --pmc summary
-    (profiles/traffic.json), or None."""
+    """HBM bytes per step-kernel launch from the committed rocprofv3 --pmc
+    measurement (profiles/traffic.json: FETCH_SIZE x 2 + WRITE_SIZE x 1,
+    the factors a plain copy of known size gives), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             data = json.load(f)
-        return data[name]["collide_bytes_per_launch"]
+        return data[name]["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -76,6 +77,24 @@ def cpu_baseline(name, seconds):
             "sample": "%s, %d steps after 1 warm-up step, FP64 oracle (C, OpenMP)" % (name, steps)}
 
 
+def copy_probe(mib):
+    """Time a device-to-device copy of mib MiB (torch, 20 repetitions)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    print(json.dumps({"copy_MiB": mib, "ms": round(ms, 4), "GB/s_read_plus_write": round(2 * n * 4 / ms / 1e6, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +103,11 @@ def main():
     ap.add_argument("--workload", default="c1_bubble128_fp32_bgk", choices=sorted(workloads.WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="length of the CPU baseline sample (0 disables it)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only time a plain device copy of MIB MiB (the reference point for the rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE scale factors and an achievable-bandwidth figure)")
+    ap.add_argument("--kernel", default="auto", choices=("auto", "onepass", "twopass"),
+                    help="step kernel (default: one-pass where available)")
     ap.add_argument("--tile", default="", help="fix the one-pass tile, e.g. 64x4 (default: autotuned)")
     ap.add_argument("--zseg", type=int, default=0, help="fix the z-planes per workgroup (default: autotuned)")
     args = ap.parse_args()
@@ -91,12 +115,15 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     d, walls, rho, u, descr = workloads.workload(args.workload)
     dom = txg.Domain(d)
-    if args.tile or args.zseg:
+    if args.tile or args.zseg or args.kernel != "auto":
         tile = tuple(int(v) for v in args.tile.split("x")) if args.tile else (0, 0)
-        dom.tune("auto", tile, args.zseg)
+        dom.tune(args.kernel, tile, args.zseg)
     dom.set_walls(walls)
     dom.set_state(rho, u)
     dom.init()
