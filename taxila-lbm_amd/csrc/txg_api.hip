@@ -25,6 +25,11 @@ namespace {
 
 thread_local std::string g_create_error;
 
+__global__ void __launch_bounds__(256) k_to_float(float* __restrict__ out, const double* __restrict__ in, long n) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
 // out[k] = sum of the nblocks partials of quantity k of k_sums (FP64, one
 // block per quantity, fixed tree order, so the sums are deterministic)
 __global__ void __launch_bounds__(256) k_sums_final(const double* __restrict__ part, int nblocks, int nv,
@@ -199,7 +204,7 @@ struct txg_handle {
   int have_old = 0;
   Params<double> p64{};
   Params<float> p32{};
-  std::vector<uint32_t> hmask;
+  std::vector<uint8_t> hsolid;  // wall nodes, for the exports' zeroing
   // event timing of the step passes (txg_profile_enable)
   int prof = 0;
   int ev_next = 0;
@@ -237,6 +242,11 @@ struct txg_handle {
   template <class T>
   bool alloc(T** p, size_t bytes) {
     return hipMallocAsync((void**)p, bytes, stream) == hipSuccess;
+  }
+  // device -> host copy ordered on the handle's stream, then waited for
+  int download(void* dst, const void* src, size_t bytes, const char* what) {
+    int rc = hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream), what);
+    return rc ? rc : hip(hipStreamSynchronize(stream), what);
   }
   template <class T>
   void release(T*& p) {
@@ -519,7 +529,9 @@ int txg_create(const txg_desc* d, txg_handle** out) {
   if (h->prec == TXG_FP32_FP16)
     for (int m = 0; m < h->C; m++) h->p32.shift0[m] = (float)d->rho0[m];
 
-  if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
+  // non-blocking: the handle's stream never waits for the legacy default
+  // stream (nor it for the handle), so handles and batches overlap
+  if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return bad(TXG_ERR_HIP, "no HIP device / stream creation failed");
   }
@@ -545,7 +557,8 @@ int txg_create(const txg_desc* d, txg_handle** out) {
     for (int n = 0; n < Q; n++) wq[n] = h->D == 3 ? Lat<3>::w[n] : Lat<2>::w[n];
     if (h->prec == TXG_FP64) {
       std::vector<double> s((size_t)h->C * Q, 0.0);
-      (void)hipMemcpy(h->shift, s.data(), s.size() * 8, hipMemcpyHostToDevice);
+      (void)hipMemcpyAsync(h->shift, s.data(), s.size() * 8, hipMemcpyHostToDevice, h->stream);
+      (void)hipStreamSynchronize(h->stream);
     } else {
       std::vector<float> s((size_t)h->C * Q, 0.0f);
       for (int m = 0; m < h->C; m++)
@@ -553,7 +566,8 @@ int txg_create(const txg_desc* d, txg_handle** out) {
           s[m * Q + n] = (float)wq[n] * h->p32.shift0[m];
           h->p32.shiftw[m][h->D == 3 ? wclass<3>(n) : wclass<2>(n)] = s[m * Q + n];
         }
-      (void)hipMemcpy(h->shift, s.data(), s.size() * 4, hipMemcpyHostToDevice);
+      (void)hipMemcpyAsync(h->shift, s.data(), s.size() * 4, hipMemcpyHostToDevice, h->stream);
+      (void)hipStreamSynchronize(h->stream);
     }
   }
   if (hipStreamSynchronize(h->stream) != hipSuccess) {
@@ -597,11 +611,9 @@ int txg_set_walls(txg_handle* h, const double* walls) {
   }
   (void)hipFreeAsync(dw, h->stream);
   if (rc) return rc;
-  // the host keeps the masks for the solid test of the exports
-  h->hmask.assign(h->Np, 0u);
-  rc = h->hip(hipMemcpyAsync(h->hmask.data(), h->mask, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream),
-              "mask download");
-  if (rc) return rc;
+  // the host keeps the wall flags for the exports' zeroing of walls
+  h->hsolid.resize(N);
+  for (int i = 0; i < N; i++) h->hsolid[i] = walls[i] != 0.0;
   rc = h->hip(hipStreamSynchronize(h->stream), "walls sync");
   if (rc) return rc;
   h->walls_set = 1;
@@ -616,6 +628,9 @@ int txg_set_state(txg_handle* h, const double* rho, const double* u) {
   double* tmp = nullptr;
   size_t nr = (size_t)h->N * h->C, nu = (size_t)h->N * h->C * h->D;
   if (!h->alloc(&tmp, std::max(nr, nu) * 8)) return h->fail(TXG_ERR_NOMEM, "staging allocation failed");
+  // stream-ordered: rho staged and converted, then u through the same
+  // staging buffer; one synchronisation at the end (the caller's arrays
+  // must outlive the copies)
   int rc = h->hip(hipMemcpyAsync(tmp, rho, nr * 8, hipMemcpyHostToDevice, h->stream), "rho upload");
   if (!rc) {
     if (R == 8)
@@ -624,24 +639,24 @@ int txg_set_state(txg_handle* h, const double* rho, const double* u) {
     else
       hipLaunchKernelGGL(k_fill_rho<float>, dim3(cdiv(h->N, 256)), dim3(256), 0, h->stream, (float*)h->rho, tmp,
                          h->mask, h->N, h->Np, h->C);
-    rc = h->hip(hipStreamSynchronize(h->stream), "rho fill");
+    rc = h->hip(hipGetLastError(), "rho fill");
   }
+  if (!rc && !h->u_ic && !h->alloc(&h->u_ic, nu * R)) rc = h->fail(TXG_ERR_NOMEM, "u allocation failed");
   if (!rc) {
-    if (!h->u_ic && !h->alloc(&h->u_ic, nu * R)) rc = h->fail(TXG_ERR_NOMEM, "u allocation failed");
-  }
-  if (!rc) {
-    std::vector<double> uz(nu, 0.0);
-    const double* us = u ? u : uz.data();
-    if (R == 8) {
-      rc = h->hip(hipMemcpyAsync(h->u_ic, us, nu * 8, hipMemcpyHostToDevice, h->stream), "u upload");
+    if (!u) {
+      rc = h->hip(hipMemsetAsync(h->u_ic, 0, nu * R, h->stream), "u clear");
+    } else if (R == 8) {
+      rc = h->hip(hipMemcpyAsync(h->u_ic, u, nu * 8, hipMemcpyHostToDevice, h->stream), "u upload");
     } else {
-      std::vector<float> uf(nu);
-      for (size_t i = 0; i < nu; i++) uf[i] = (float)us[i];
-      rc = h->hip(hipMemcpyAsync(h->u_ic, uf.data(), nu * 4, hipMemcpyHostToDevice, h->stream), "u upload");
+      rc = h->hip(hipMemcpyAsync(tmp, u, nu * 8, hipMemcpyHostToDevice, h->stream), "u upload");
+      if (!rc) {
+        hipLaunchKernelGGL(k_to_float, dim3(cdiv((long)nu, 256)), dim3(256), 0, h->stream, (float*)h->u_ic, tmp,
+                           (long)nu);
+        rc = h->hip(hipGetLastError(), "u conversion");
+      }
     }
-    // the host vectors above must outlive the copies
-    if (!rc) rc = h->hip(hipStreamSynchronize(h->stream), "u upload sync");
   }
+  if (!rc) rc = h->hip(hipStreamSynchronize(h->stream), "state upload sync");
   h->release(tmp);
   if (rc) return rc;
   h->state_set = 1;
@@ -913,8 +928,9 @@ int txg_profile_read(txg_handle* h, double* ms, int32_t* n) {
 int txg_get_mask(txg_handle* h, uint32_t* mask) {
   if (!h || !mask) return TXG_ERR_ARG;
   if (!h->walls_set) return h->fail(TXG_ERR_STATE, "txg_set_walls has not been called");
-  memcpy(mask, h->hmask.data(), (size_t)h->N * sizeof(uint32_t));
-  return TXG_OK;
+  int rc = h->hip(hipStreamSynchronize(h->stream), "mask sync");
+  if (rc) return rc;
+  return h->download(mask, h->mask, (size_t)h->N * sizeof(uint32_t), "mask download");
 }
 
 int txg_sync(txg_handle* h) {
@@ -928,12 +944,12 @@ int txg_get_fi(txg_handle* h, double* fi) {
   if (rc) return rc;
   const size_t N = h->N, C = h->C, Q = h->Q;
   std::vector<double> tmp(N * C * Q);
-  rc = h->hip(hipMemcpy(tmp.data(), export_ptrs(h).f, tmp.size() * 8, hipMemcpyDeviceToHost), "fi download");
+  rc = h->download(tmp.data(), export_ptrs(h).f, tmp.size() * 8, "fi download");
   if (rc) return rc;
   for (size_t m = 0; m < C; m++)
     for (size_t n = 0; n < Q; n++) {
       const double* src = tmp.data() + (m * Q + n) * N;
-      for (size_t i = 0; i < N; i++) fi[(i * Q + n) * C + m] = h->hmask[i] & kSelfSolid ? 0.0 : src[i];
+      for (size_t i = 0; i < N; i++) fi[(i * Q + n) * C + m] = h->hsolid[i] ? 0.0 : src[i];
     }
   return TXG_OK;
 }
@@ -945,12 +961,12 @@ int txg_get_moments(txg_handle* h, double* rho, double* flux, double* forces) {
   const size_t N = h->N, C = h->C, D = h->D;
   ExportPtrs e = export_ptrs(h);
   std::vector<double> r(N * C), up(N * D), F(N * C * D);
-  rc = h->hip(hipMemcpy(r.data(), e.rho, r.size() * 8, hipMemcpyDeviceToHost), "rho download");
-  if (!rc) rc = h->hip(hipMemcpy(up.data(), e.up, up.size() * 8, hipMemcpyDeviceToHost), "u download");
-  if (!rc) rc = h->hip(hipMemcpy(F.data(), e.F, F.size() * 8, hipMemcpyDeviceToHost), "F download");
+  rc = h->download(r.data(), e.rho, r.size() * 8, "rho download");
+  if (!rc) rc = h->download(up.data(), e.up, up.size() * 8, "u download");
+  if (!rc) rc = h->download(F.data(), e.F, F.size() * 8, "F download");
   if (rc) return rc;
   for (size_t i = 0; i < N; i++) {
-    const bool solid = h->hmask[i] & kSelfSolid;
+    const bool solid = h->hsolid[i];
     for (size_t m = 0; m < C; m++) {
       if (rho) rho[i * C + m] = solid ? 0.0 : r[m * N + i];
       for (size_t d = 0; d < D; d++) {
@@ -969,14 +985,14 @@ int txg_diagnostics(txg_handle* h, double* rhot, double* prs, double* velt) {
   const size_t N = h->N, C = h->C, D = h->D;
   ExportPtrs e = export_ptrs(h);
   std::vector<double> r(N * C), F(N * C * D), j(N * C * D);
-  rc = h->hip(hipMemcpy(r.data(), e.rho, r.size() * 8, hipMemcpyDeviceToHost), "rho download");
-  if (!rc) rc = h->hip(hipMemcpy(F.data(), e.F, F.size() * 8, hipMemcpyDeviceToHost), "F download");
-  if (!rc) rc = h->hip(hipMemcpy(j.data(), e.j, j.size() * 8, hipMemcpyDeviceToHost), "j download");
+  rc = h->download(r.data(), e.rho, r.size() * 8, "rho download");
+  if (!rc) rc = h->download(F.data(), e.F, F.size() * 8, "F download");
+  if (!rc) rc = h->download(j.data(), e.j, j.size() * 8, "j download");
   if (rc) return rc;
   const txg_desc& d = h->d;
   // FlowUpdateDiagnosticsD3/D2 (lbm_flow.F90:654-758), psi = rho
   for (size_t i = 0; i < N; i++) {
-    if (h->hmask[i] & kSelfSolid) {
+    if (h->hsolid[i]) {
       if (rhot) rhot[i] = 0.0;
       if (prs) prs[i] = 0.0;
       if (velt)
@@ -1013,9 +1029,8 @@ int txg_reduce(txg_handle* h, double* out) {
   rc = h->hip(hipStreamSynchronize(h->stream), "reduce sync");
   if (rc) return rc;
   // mass and momentum are the leading C + C*D flow sums
-  return h->hip(hipMemcpy(out, h->sums + (size_t)kSumBlocks * h->ks.nsums, (size_t)(h->C + h->C * h->D) * 8,
-                          hipMemcpyDeviceToHost),
-                "reduce download");
+  return h->download(out, h->sums + (size_t)kSumBlocks * h->ks.nsums, (size_t)(h->C + h->C * h->D) * 8,
+                     "reduce download");
 }
 
 int txg_flow_sums(txg_handle* h, double* out) {
@@ -1024,9 +1039,8 @@ int txg_flow_sums(txg_handle* h, double* out) {
   if (rc) return rc;
   rc = h->hip(hipStreamSynchronize(h->stream), "flow sums sync");
   if (rc) return rc;
-  return h->hip(hipMemcpy(out, h->sums + (size_t)kSumBlocks * h->ks.nsums, (size_t)h->ks.nsums * 8,
-                          hipMemcpyDeviceToHost),
-                "flow sums download");
+  return h->download(out, h->sums + (size_t)kSumBlocks * h->ks.nsums, (size_t)h->ks.nsums * 8,
+                     "flow sums download");
 }
 
 int txg_delta_norm(txg_handle* h, double* norm) {
@@ -1042,7 +1056,7 @@ int txg_delta_norm(txg_handle* h, double* norm) {
   hipLaunchKernelGGL(k_delta, dim3(nb), dim3(256), 0, h->stream, h->rho_old, e.rho, (int)n, h->have_old ? 0 : 1,
                      h->red);
   std::vector<double> part(nb);
-  rc = h->hip(hipMemcpy(part.data(), h->red, nb * 8, hipMemcpyDeviceToHost), "delta download");
+  rc = h->download(part.data(), h->red, nb * 8, "delta download");
   if (rc) return rc;
   double mx = 0.0;
   for (double v : part) mx = std::max(mx, v);
@@ -1181,23 +1195,28 @@ int write_table(txg_batch* b, const std::vector<txg_handle*>& set) {
   int rc = b->hip(hipStreamSynchronize(b->stream), "batch sync");
   if (rc) return rc;
   if (host.size() > b->dmem_cap) {
-    (void)hipFree(b->dmem);
+    if (b->dmem) (void)hipFreeAsync(b->dmem, b->stream);
     b->dmem = nullptr;
     b->dmem_cap = 0;
-    if (hipMalloc(&b->dmem, host.size()) != hipSuccess) return b->fail(TXG_ERR_NOMEM, "batch table allocation");
+    if (hipMallocAsync(&b->dmem, host.size(), b->stream) != hipSuccess)
+      return b->fail(TXG_ERR_NOMEM, "batch table allocation");
     b->dmem_cap = host.size();
   }
   if (prefix.size() * sizeof(int) > b->dprefix_cap) {
-    (void)hipFree(b->dprefix);
+    if (b->dprefix) (void)hipFreeAsync(b->dprefix, b->stream);
     b->dprefix = nullptr;
     b->dprefix_cap = 0;
-    if (hipMalloc(&b->dprefix, prefix.size() * sizeof(int)) != hipSuccess)
+    if (hipMallocAsync((void**)&b->dprefix, prefix.size() * sizeof(int), b->stream) != hipSuccess)
       return b->fail(TXG_ERR_NOMEM, "batch table allocation");
     b->dprefix_cap = prefix.size() * sizeof(int);
   }
-  rc = b->hip(hipMemcpy(b->dmem, host.data(), host.size(), hipMemcpyHostToDevice), "batch table upload");
-  if (!rc) rc = b->hip(hipMemcpy(b->dprefix, prefix.data(), prefix.size() * sizeof(int), hipMemcpyHostToDevice),
-                       "batch table upload");
+  rc = b->hip(hipMemcpyAsync(b->dmem, host.data(), host.size(), hipMemcpyHostToDevice, b->stream),
+              "batch table upload");
+  if (!rc)
+    rc = b->hip(hipMemcpyAsync(b->dprefix, prefix.data(), prefix.size() * sizeof(int), hipMemcpyHostToDevice,
+                               b->stream),
+                "batch table upload");
+  if (!rc) rc = b->hip(hipStreamSynchronize(b->stream), "batch table upload");
   if (rc) return rc;
   b->table = set;
   b->flip = 0;
@@ -1397,11 +1416,14 @@ int txg_batch_flow_sums(txg_batch* b, double* out, int32_t stride) {
   if (stride < nv) return b->fail(TXG_ERR_ARG, "stride %d < %d flow sums per member", stride, nv);
   const size_t n = b->mem.size();
   if (n * nv > b->dsums_len) {
-    (void)hipFree(b->dsums);
+    if (b->dsums) (void)hipFreeAsync(b->dsums, b->stream);
     b->dsums = nullptr;
     b->dsums_len = 0;
-    if (hipMalloc(&b->dsums, n * nv * sizeof(double)) != hipSuccess)
+    if (hipMallocAsync((void**)&b->dsums, n * nv * sizeof(double), b->stream) != hipSuccess)
       return b->fail(TXG_ERR_NOMEM, "batch sums allocation");
+    // the members' streams write into it
+    int rc0 = b->hip(hipStreamSynchronize(b->stream), "batch sums allocation");
+    if (rc0) return rc0;
     b->dsums_len = n * nv;
   }
   // each member's sums on its own stream (they overlap), then gather
@@ -1476,9 +1498,10 @@ int txg_batch_destroy(txg_batch* b) {
   if (!b) return TXG_OK;
   (void)hipStreamSynchronize(b->stream);
   for (txg_handle* h : b->mem) h->batch = nullptr;
-  (void)hipFree(b->dmem);
-  (void)hipFree(b->dprefix);
-  (void)hipFree(b->dsums);
+  if (b->dmem) (void)hipFreeAsync(b->dmem, b->stream);
+  if (b->dprefix) (void)hipFreeAsync(b->dprefix, b->stream);
+  if (b->dsums) (void)hipFreeAsync(b->dsums, b->stream);
+  (void)hipStreamSynchronize(b->stream);
   (void)hipEventDestroy(b->done);
   (void)hipStreamDestroy(b->stream);
   delete b;

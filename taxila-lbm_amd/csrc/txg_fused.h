@@ -262,9 +262,23 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
     return zc >= 0;
   };
 
-  // pull the column at rows (xr, yr, zr); density (Dirichlet-overridden) to rho
-  auto pull_col = [&](const uint32_t (&xr)[3], const uint32_t (&yr)[3], const uint32_t (&zr)[3], R (&f)[C][Q],
-                      R (&rho)[C]) {
+  // link masks of the own column and of ring column k at plane zz (walls
+  // for cells beyond a non-periodic face), loaded ahead of their use
+  auto own_mask = [&](int zz) -> uint32_t {
+    uint32_t zr[3];
+    if (!oval || !zrows(zz, zr)) return kSelfSolid;
+    return b.mask[(ox[1] + oy[1] + zr[1]) / SB];
+  };
+  auto ring_mask = [&](int zz, int k) -> uint32_t {
+    uint32_t zr[3];
+    if (rcol[k] < 0 || !rval[k] || !zrows(zz, zr)) return kSelfSolid;
+    return b.mask[(rx[k][1] + ry[k][1] + zr[1]) / SB];
+  };
+
+  // pull the column at rows (xr, yr, zr) with link mask mk; density
+  // (Dirichlet-overridden) to rho
+  auto pull_col = [&](const uint32_t (&xr)[3], const uint32_t (&yr)[3], const uint32_t (&zr)[3], uint32_t mk,
+                      R (&f)[C][Q], R (&rho)[C]) {
     RowsB cr;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -272,8 +286,6 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
       cr.y[d] = yr[d];
       cr.z[d] = zr[d];
     }
-    const uint32_t node = (cr.x[1] + cr.y[1] + cr.z[1]) / SB;
-    const uint32_t mk = b.mask[node];
 #pragma unroll
     for (int m = 0; m < C; m++) rho[m] = (R)NAN;
     if (mk & kSelfSolid) return;
@@ -296,38 +308,48 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
 
   auto slot_of = [&](int zz) { return (zz - (z0 - RR) + NPL * 4096) % NPL; };
 
-  // phase A for plane zz: own column into f (kept for phase B), ring columns
-  auto phase_a = [&](int zz, R (&f)[C][Q]) {
+  // ring-column masks of one plane
+  struct RingMasks {
+    uint32_t m[NRP > 0 ? NRP : 1];
+  };
+  auto ring_masks = [&](int zz) {
+    RingMasks r;
+#pragma unroll
+    for (int k = 0; k < NRP; k++) r.m[k] = ring_mask(zz, k);
+    return r;
+  };
+
+  // phase A for plane zz: own column into f (kept for phase B), ring
+  // columns; mo and mr are the plane's masks (loaded one step earlier)
+  auto phase_a = [&](int zz, R (&f)[C][Q], uint32_t mo, const RingMasks& mr) {
     const int slot = slot_of(zz);
     uint32_t zr[3];
-    const bool zval = zrows(zz, zr);
+    zrows(zz, zr);
     R rho[C];
-#pragma unroll
-    for (int m = 0; m < C; m++) rho[m] = (R)NAN;
-    if (oval && zval) pull_col(ox, oy, zr, f, rho);
+    pull_col(ox, oy, zr, mo, f, rho);
 #pragma unroll
     for (int m = 0; m < C; m++) srho[(slot * C + m) * NCOL + ocol] = rho[m];
 #pragma unroll
     for (int k = 0; k < NRP; k++) {
       if (rcol[k] < 0) continue;
       R rr[C];
-#pragma unroll
-      for (int m = 0; m < C; m++) rr[m] = (R)NAN;
-      if (rval[k] && zval) {
-        R fr[C][Q];
-        pull_col(rx[k], ry[k], zr, fr, rr);
-      }
+      R fr[C][Q];
+      pull_col(rx[k], ry[k], zr, mr.m[k], fr, rr);
 #pragma unroll
       for (int m = 0; m < C; m++) srho[(slot * C + m) * NCOL + rcol[k]] = rr[m];
     }
   };
+  auto phase_a_now = [&](int zz, R (&f)[C][Q]) -> uint32_t {
+    const uint32_t mo = own_mask(zz);
+    phase_a(zz, f, mo, ring_masks(zz));
+    return mo;
+  };
 
   // phase B for plane z with the own column's populations f
-  auto phase_b = [&](int z, R (&f)[C][Q]) {
+  auto phase_b = [&](int z, R (&f)[C][Q], uint32_t mk) {
     if (!active) return;
     set_z<3, 1, R>(p, z, nb);
     const int node = nb.xw[1] + nb.yw[1] + nb.zw[1];
-    const uint32_t mk = b.mask[node];
     if (mk & kSelfSolid) return;
     if (mk & (0x3fu << kFaceShift)) apply_dirichlet<3, C, R>(p, mk, f);
     LdsRho<R, C, RR, EXW, EYW> acc;
@@ -388,11 +410,16 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
     // registers too few for a ring of RR+1 planes: phase B re-pulls plane z
     // (read RR planes earlier by this workgroup, so mostly from L2)
     R f[C][Q];
-    for (int zz = z0 - RR; zz < z0 + RR; zz++) phase_a(zz, f);
+    for (int zz = z0 - RR; zz < z0 + RR; zz++) phase_a_now(zz, f);
+    uint32_t mo = own_mask(z0 + RR);
+    RingMasks mr = ring_masks(z0 + RR);
     for (int z = z0; z < z1; z++) {
-      phase_a(z + RR, f);
+      const uint32_t mo2 = own_mask(z + RR + 1);
+      const RingMasks mr2 = ring_masks(z + RR + 1);
+      const uint32_t mz = own_mask(z);
+      phase_a(z + RR, f, mo, mr);
       __syncthreads();
-      if (active) {
+      if (active && !(mz & kSelfSolid)) {
         uint32_t zr[3];
         zrows(z, zr);
         RowsB cr;
@@ -402,45 +429,60 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
           cr.y[d] = oy[d];
           cr.z[d] = zr[d];
         }
-        const uint32_t mk = b.mask[(cr.x[1] + cr.y[1] + cr.z[1]) / SB];
-        if (!(mk & kSelfSolid)) pull_buf<C, S, R>(p, rin, mk, cr, plane_b, f);
+        pull_buf<C, S, R>(p, rin, mz, cr, plane_b, f);
       }
-      phase_b(z, f);
+      phase_b(z, f, mz);
       __syncthreads();
+      mo = mo2;
+      mr = mr2;
     }
   } else if constexpr (RR == 1) {
     // register ring of two planes: the populations of plane z+1 pulled in
-    // phase A are the ones phase B collides one plane later
+    // phase A are the ones phase B collides one plane later; the masks of
+    // plane z+2 are loaded one step before their phase A
     R fa[C][Q], fb[C][Q];
-    phase_a(z0 - 1, fb);  // density only
-    phase_a(z0, fa);
-    auto step = [&](int z, R (&cur)[C][Q], R (&nxt)[C][Q]) {
-      phase_a(z + 1, nxt);
+    phase_a_now(z0 - 1, fb);  // density only
+    uint32_t ma = phase_a_now(z0, fa), mb = 0;
+    uint32_t mo = own_mask(z0 + 1);
+    RingMasks mr = ring_masks(z0 + 1);
+    auto step = [&](int z, R (&cur)[C][Q], uint32_t& mcur, R (&nxt)[C][Q], uint32_t& mnxt) {
+      const uint32_t mo2 = own_mask(z + 2);
+      const RingMasks mr2 = ring_masks(z + 2);
+      phase_a(z + 1, nxt, mo, mr);
+      mnxt = mo;
       __syncthreads();
-      phase_b(z, cur);
+      phase_b(z, cur, mcur);
       __syncthreads();
+      mo = mo2;
+      mr = mr2;
     };
     for (int z = z0; z < z1; z += 2) {
-      step(z, fa, fb);
-      if (z + 1 < z1) step(z + 1, fb, fa);
+      step(z, fa, ma, fb, mb);
+      if (z + 1 < z1) step(z + 1, fb, mb, fa, ma);
     }
   } else {
     // register ring of three planes
     R f0[C][Q], f1[C][Q], f2[C][Q];
-    phase_a(z0 - 2, f2);  // density only
-    phase_a(z0 - 1, f2);  // density only
-    phase_a(z0, f0);
-    phase_a(z0 + 1, f1);
-    auto step = [&](int z, R (&cur)[C][Q], R (&nxt)[C][Q]) {
-      phase_a(z + 2, nxt);
+    phase_a_now(z0 - 2, f2);  // density only
+    phase_a_now(z0 - 1, f2);  // density only
+    uint32_t m0 = phase_a_now(z0, f0), m1 = phase_a_now(z0 + 1, f1), m2 = 0;
+    uint32_t mo = own_mask(z0 + 2);
+    RingMasks mr = ring_masks(z0 + 2);
+    auto step = [&](int z, R (&cur)[C][Q], uint32_t& mcur, R (&nxt)[C][Q], uint32_t& mnxt) {
+      const uint32_t mo2 = own_mask(z + 3);
+      const RingMasks mr2 = ring_masks(z + 3);
+      phase_a(z + 2, nxt, mo, mr);
+      mnxt = mo;
       __syncthreads();
-      phase_b(z, cur);
+      phase_b(z, cur, mcur);
       __syncthreads();
+      mo = mo2;
+      mr = mr2;
     };
     for (int z = z0; z < z1; z += 3) {
-      step(z, f0, f2);
-      if (z + 1 < z1) step(z + 1, f1, f0);
-      if (z + 2 < z1) step(z + 2, f2, f1);
+      step(z, f0, m0, f2, m2);
+      if (z + 1 < z1) step(z + 1, f1, m1, f0, m0);
+      if (z + 2 < z1) step(z + 2, f2, m2, f1, m1);
     }
   }
 }
