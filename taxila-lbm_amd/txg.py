@@ -18,7 +18,7 @@ import struct
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "build", "libtaxila_lbm.so")
+LIB_PATH = os.environ.get("TXG_LIB") or os.path.join(HERE, "build", "libtaxila_lbm.so")
 
 MAXC = 4
 MAXMIN = 16
