@@ -7,7 +7,8 @@ bit), FP32 storage and shifted-FP16 storage on the GPU from the same initial
 state and prints, for each reduced-precision mode, the max and RMS error of
 the component densities, of the velocity u = sum_m mm_m (j_m + F_m/2) /
 rho_t (FlowUpdateDiagnosticsD3, lbm_flow.F90:654-708), of the interface
-position and the relative mass change of each component over the run.
+position (bubble radius, or saturation and front position of a drainage)
+and the relative mass change of each component over the run.
 
 The interface position of a bubble is its equivalent radius: with
 phi = clip((rho_A - rho_out) / (rho_in - rho_out), 0, 1) the bubble volume
@@ -49,7 +50,17 @@ def bubble_radius(rho_a, fluid, ndims, rho_in=workloads.RHO_MAJOR, rho_out=workl
     return (3.0 * v / (4.0 * math.pi)) ** (1.0 / 3.0) if ndims == 3 else math.sqrt(v / math.pi)
 
 
-def errors(ref, out, fluid, ndims):
+def invading(rho_a, fluid, shape, rho_in=workloads.RHO_MAJOR, rho_out=workloads.RHO_MINOR):
+    """(saturation, mean x) of the invading component A: phi as for the
+    bubble radius, saturation = mean phi over pore nodes, mean x = the
+    phi-weighted x of the pore nodes (the front position of a drainage)."""
+    phi = np.clip((rho_a - rho_out) / (rho_in - rho_out), 0.0, 1.0)
+    x = np.tile(np.arange(shape[0]), int(np.prod(shape)) // shape[0])
+    w = phi[fluid]
+    return float(w.mean()), float((w * x[fluid]).sum() / max(w.sum(), 1e-300))
+
+
+def errors(ref, out, fluid, ndims, shape=None, drainage=False):
     """Error figures of `out` against `ref` (both from run())."""
     r0, u0, _, _ = ref
     r1, u1, m0, m1 = out
@@ -64,9 +75,14 @@ def errors(ref, out, fluid, ndims):
     e["u_max"] = float(du.max())
     e["u_rms"] = float(np.sqrt(np.mean(du * du)))
     e["u_ref_max"] = float(np.sqrt((u0 ** 2).sum(axis=0))[fluid].max())
-    ra, rb = bubble_radius(r0[0], fluid, ndims), bubble_radius(r1[0], fluid, ndims)
-    e["radius_ref"] = ra
-    e["radius_err"] = rb - ra
+    if drainage:
+        (sa, xa), (sb, xb) = invading(r0[0], fluid, shape), invading(r1[0], fluid, shape)
+        e["saturation_ref"], e["saturation_err"] = sa, sb - sa
+        e["front_x_ref"], e["front_x_err"] = xa, xb - xa
+    else:
+        ra, rb = bubble_radius(r0[0], fluid, ndims), bubble_radius(r1[0], fluid, ndims)
+        e["radius_ref"] = ra
+        e["radius_err"] = rb - ra
     return e
 
 
@@ -77,19 +93,21 @@ def compare(name: str, steps: int, fp16_shifts=None):
     d, walls, rho, u, _ = workloads.workload(name)
     fluid = walls == 0
     C = d.ncomponents
+    shape = (d.nx, d.ny, d.nz) if d.ndims == 3 else (d.nx, d.ny)
+    kw = dict(shape=shape, drainage=name.startswith("c2_drainage"))
     own = [d.rho0[m] for m in range(C)]
     d.precision = txg.FP64
     ref = run(d, walls, rho, u, steps)
     out = {"workload": name, "steps": steps,
            "fp64_mass_rel_change": [float((ref[3][m] - ref[2][m]) / ref[2][m]) for m in range(C)]}
     d.precision = txg.FP32
-    out["fp32"] = errors(ref, run(d, walls, rho, u, steps), fluid, d.ndims)
+    out["fp32"] = errors(ref, run(d, walls, rho, u, steps), fluid, d.ndims, **kw)
     d.precision = txg.FP32_FP16
     for s in (fp16_shifts or [None]):
         for m in range(C):
             d.rho0[m] = own[m] if s is None else s
         key = "fp16" if s is None else "fp16_shift%g" % s
-        out[key] = errors(ref, run(d, walls, rho, u, steps), fluid, d.ndims)
+        out[key] = errors(ref, run(d, walls, rho, u, steps), fluid, d.ndims, **kw)
         out[key]["shift"] = [d.rho0[m] for m in range(C)]
     return out
 

@@ -131,12 +131,31 @@ def drainage_desc(n, precision=txg.FP32, isotropy_order=8, shape=None):
                          rho0=[RHO_MINOR, RHO_MINOR])
 
 
-def drainage_ic(n, seed, shape=None, inlet_layer=3):
+def connected_to_inlet(walls, shape):
+    """Pore nodes 6-connected to a pore node of the x = 0 plane."""
+    from scipy import ndimage
+    nx, ny, nz = shape
+    pore = (walls.reshape(nz, ny, nx) == 0)
+    lab, _ = ndimage.label(pore)
+    keep = np.unique(lab[:, :, 0])
+    keep = keep[keep > 0]
+    return np.isin(lab, keep).reshape(-1)
+
+
+def drainage_ic(n, seed, shape=None, inlet_layer=3, reservoir=3):
     """(walls, rho, u): porosity-0.40 pack of radii U[6, 12] (mineral per
     sphere with probability 1/2, non-periodic), filled with B except an
-    inlet layer of A."""
+    inlet layer of A.  The `reservoir` planes next to the inlet and outlet
+    faces are kept open (a density boundary node inside a pore throat, with
+    bounce-back links on most sides, drives an ever faster jet through it),
+    and pores not connected to the inlet are filled with mineral 1 (they
+    take no part in the drainage)."""
     shape = shape or (n, n, n)
     walls = sphere_pack(shape, 0.40, 6.0, 12.0, seed, minerals=(1.0, 2.0), periodic=False)
+    nx = shape[0]
+    xs = np.tile(np.arange(nx), int(np.prod(shape)) // nx)
+    walls = np.where((xs < reservoir) | (xs >= nx - reservoir), 0.0, walls)
+    walls = np.where((walls == 0) & ~connected_to_inlet(walls, shape), 1.0, walls)
     nx = shape[0]
     x = np.tile(np.arange(nx), int(np.prod(shape)) // nx)
     a = np.where(x < inlet_layer, RHO_MAJOR, RHO_MINOR)
