@@ -183,11 +183,17 @@ struct FusedShape {
 };
 
 // one work item of the one-pass step: the TX x TY column tile at (x0, y0)
-// over planes z0..z1-1.  srho is the workgroup's LDS density ring.  Ends
-// with a workgroup barrier when it uses LDS, so items can follow each other.
+// over planes z0..z1-1, marched upwards (dir = +1) or downwards (dir = -1).
+// srho is the workgroup's LDS density ring.  Ends with a workgroup barrier
+// when it uses LDS, so items can follow each other.
+//
+// Neighbouring z-segments of a tile march in opposite directions: a segment
+// boundary is then where both segments start (or both end), so the halo
+// planes one segment reads beyond its range are read by the other at about
+// the same time, and the second read hits L2 instead of HBM.
 template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
 __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>& b, int x0, int y0, int z0, int z1,
-                                            R* srho) {
+                                            int dir, R* srho) {
   using L = Lat<3>;
   using Sh = FusedShape<NFF, TX, TY>;
   constexpr int Q = L::Q;
@@ -409,15 +415,17 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
   } else if constexpr (!KEEP) {
     // registers too few for a ring of RR+1 planes: phase B re-pulls plane z
     // (read RR planes earlier by this workgroup, so mostly from L2)
+    const int zs = dir > 0 ? z0 : z1 - 1, nzs = z1 - z0;
     R f[C][Q];
-    for (int zz = z0 - RR; zz < z0 + RR; zz++) phase_a_now(zz, f);
-    uint32_t mo = own_mask(z0 + RR);
-    RingMasks mr = ring_masks(z0 + RR);
-    for (int z = z0; z < z1; z++) {
-      const uint32_t mo2 = own_mask(z + RR + 1);
-      const RingMasks mr2 = ring_masks(z + RR + 1);
+    for (int k = -RR; k < RR; k++) phase_a_now(zs + k * dir, f);
+    uint32_t mo = own_mask(zs + RR * dir);
+    RingMasks mr = ring_masks(zs + RR * dir);
+    for (int i = 0; i < nzs; i++) {
+      const int z = zs + i * dir;
+      const uint32_t mo2 = own_mask(z + (RR + 1) * dir);
+      const RingMasks mr2 = ring_masks(z + (RR + 1) * dir);
       const uint32_t mz = own_mask(z);
-      phase_a(z + RR, f, mo, mr);
+      phase_a(z + RR * dir, f, mo, mr);
       __syncthreads();
       if (active && !(mz & kSelfSolid)) {
         uint32_t zr[3];
@@ -437,18 +445,19 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
       mr = mr2;
     }
   } else if constexpr (RR == 1) {
-    // register ring of two planes: the populations of plane z+1 pulled in
-    // phase A are the ones phase B collides one plane later; the masks of
-    // plane z+2 are loaded one step before their phase A
+    // register ring of two planes: the populations of the next plane pulled
+    // in phase A are the ones phase B collides one plane later; the masks of
+    // the plane after are loaded one step before their phase A
+    const int zs = dir > 0 ? z0 : z1 - 1, nzs = z1 - z0;
     R fa[C][Q], fb[C][Q];
-    phase_a_now(z0 - 1, fb);  // density only
-    uint32_t ma = phase_a_now(z0, fa), mb = 0;
-    uint32_t mo = own_mask(z0 + 1);
-    RingMasks mr = ring_masks(z0 + 1);
+    phase_a_now(zs - dir, fb);  // density only
+    uint32_t ma = phase_a_now(zs, fa), mb = 0;
+    uint32_t mo = own_mask(zs + dir);
+    RingMasks mr = ring_masks(zs + dir);
     auto step = [&](int z, R (&cur)[C][Q], uint32_t& mcur, R (&nxt)[C][Q], uint32_t& mnxt) {
-      const uint32_t mo2 = own_mask(z + 2);
-      const RingMasks mr2 = ring_masks(z + 2);
-      phase_a(z + 1, nxt, mo, mr);
+      const uint32_t mo2 = own_mask(z + 2 * dir);
+      const RingMasks mr2 = ring_masks(z + 2 * dir);
+      phase_a(z + dir, nxt, mo, mr);
       mnxt = mo;
       __syncthreads();
       phase_b(z, cur, mcur);
@@ -456,22 +465,23 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
       mo = mo2;
       mr = mr2;
     };
-    for (int z = z0; z < z1; z += 2) {
-      step(z, fa, ma, fb, mb);
-      if (z + 1 < z1) step(z + 1, fb, mb, fa, ma);
+    for (int i = 0; i < nzs; i += 2) {
+      step(zs + i * dir, fa, ma, fb, mb);
+      if (i + 1 < nzs) step(zs + (i + 1) * dir, fb, mb, fa, ma);
     }
   } else {
     // register ring of three planes
+    const int zs = dir > 0 ? z0 : z1 - 1, nzs = z1 - z0;
     R f0[C][Q], f1[C][Q], f2[C][Q];
-    phase_a_now(z0 - 2, f2);  // density only
-    phase_a_now(z0 - 1, f2);  // density only
-    uint32_t m0 = phase_a_now(z0, f0), m1 = phase_a_now(z0 + 1, f1), m2 = 0;
-    uint32_t mo = own_mask(z0 + 2);
-    RingMasks mr = ring_masks(z0 + 2);
+    phase_a_now(zs - 2 * dir, f2);  // density only
+    phase_a_now(zs - dir, f2);      // density only
+    uint32_t m0 = phase_a_now(zs, f0), m1 = phase_a_now(zs + dir, f1), m2 = 0;
+    uint32_t mo = own_mask(zs + 2 * dir);
+    RingMasks mr = ring_masks(zs + 2 * dir);
     auto step = [&](int z, R (&cur)[C][Q], uint32_t& mcur, R (&nxt)[C][Q], uint32_t& mnxt) {
-      const uint32_t mo2 = own_mask(z + 3);
-      const RingMasks mr2 = ring_masks(z + 3);
-      phase_a(z + 2, nxt, mo, mr);
+      const uint32_t mo2 = own_mask(z + 3 * dir);
+      const RingMasks mr2 = ring_masks(z + 3 * dir);
+      phase_a(z + 2 * dir, nxt, mo, mr);
       mnxt = mo;
       __syncthreads();
       phase_b(z, cur, mcur);
@@ -479,10 +489,10 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
       mo = mo2;
       mr = mr2;
     };
-    for (int z = z0; z < z1; z += 3) {
-      step(z, f0, m0, f2, m2);
-      if (z + 1 < z1) step(z + 1, f1, m1, f0, m0);
-      if (z + 2 < z1) step(z + 2, f2, m2, f1, m1);
+    for (int i = 0; i < nzs; i += 3) {
+      step(zs + i * dir, f0, m0, f2, m2);
+      if (i + 1 < nzs) step(zs + (i + 1) * dir, f1, m1, f0, m0);
+      if (i + 2 < nzs) step(zs + (i + 2) * dir, f2, m2, f1, m1);
     }
   }
 }
@@ -491,11 +501,13 @@ template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(TXG_FUSED_WAVES_PER_EU)))
 k_fused3(Params<R> p, Bufs<S, R> b, FusedGeom g) {
   __shared__ R srho[FusedShape<NFF, TX, TY>::template lds_len<C>()];
+  // segments of one tile are consecutive work items (same XCD, so the
+  // halo planes two segments share are served by one L2)
   const int lb = xcd_remap(blockIdx.x, g.nblocks);
-  const int tile = lb % g.ntiles, seg = lb / g.ntiles;
+  const int seg = lb % g.nseg, tile = lb / g.nseg;
   const int z0 = seg * g.zseg;
   fused3_item<C, NFF, FAST, S, R, TX, TY>(p, b, (tile % g.ntx) * TX, (tile / g.ntx) * TY, z0,
-                                          min(p.nz, z0 + g.zseg), srho);
+                                          min(p.nz, z0 + g.zseg), (seg & 1) ? 1 : -1, srho);
 }
 
 // one member domain of a batched one-pass launch: its parameters, its two
@@ -534,10 +546,11 @@ k_fused3_batch(const BatchMember<S, R>* __restrict__ mem, const int* __restrict_
     }
     const BatchMember<S, R>& m = mem[a];
     const int lb = it - prefix[a];
-    const int tile = lb % m.g.ntiles, seg = lb / m.g.ntiles;
+    const int seg = lb % m.g.nseg, tile = lb / m.g.nseg;
     const int z0 = seg * m.g.zseg;
     fused3_item<C, NFF, FAST, S, R, TX, TY>(m.p, m.b[m.parity ^ flip], (tile % m.g.ntx) * TX,
-                                            (tile / m.g.ntx) * TY, z0, min(m.p.nz, z0 + m.g.zseg), srho);
+                                            (tile / m.g.ntx) * TY, z0, min(m.p.nz, z0 + m.g.zseg),
+                                            (seg & 1) ? 1 : -1, srho);
   }
 }
 
