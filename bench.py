@@ -36,7 +36,12 @@ DTYPE = {txg.FP64: "f64", txg.FP32: "f32", txg.FP32_FP16: "f32 (f16 storage)"}
 
 
 def b_alg(desc):
-    return 2 * desc.Q * desc.ncomponents * STORAGE_BYTES[desc.precision] + 1
+    """Algorithmic bytes per node-step: every population read once and
+    written once plus the one-byte tag; with FP32 or FP16 storage the rest
+    population is kept one format wider (double, float)."""
+    s = STORAGE_BYTES[desc.precision]
+    rest = {txg.FP64: 8, txg.FP32: 8, txg.FP32_FP16: 4}[desc.precision]
+    return 2 * desc.ncomponents * ((desc.Q - 1) * s + rest) + 1
 
 
 def traffic_for(name):

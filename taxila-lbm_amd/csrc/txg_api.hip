@@ -83,14 +83,17 @@ __global__ void k_fill_rho(R* __restrict__ rho, const double* __restrict__ src, 
 
 // AoS FP64 fi(1:C,0:b,node) -> SoA storage
 template <class S, class R>
-__global__ void k_load_fi(S* __restrict__ f, const double* __restrict__ src, int N, int Np, int C, int Q,
-                          const R* __restrict__ shift) {
+__global__ void k_load_fi(S* __restrict__ f, typename Rest<S>::type* __restrict__ f0, const double* __restrict__ src,
+                          int N, int Np, int C, int Q, const R* __restrict__ shift) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= N) return;
   for (int n = 0; n < Q; n++)
     for (int m = 0; m < C; m++) {
-      R v = (R)src[((size_t)i * Q + n) * C + m];
-      st<S, R>(f + (size_t)(m * Q + n) * Np, i, v, shift[m * Q + n]);
+      const double v = src[((size_t)i * Q + n) * C + m];
+      if (n == 0 && wide_rest<S>())
+        f0[(size_t)m * Np + i] = (typename Rest<S>::type)v;  // the rest population at its own width
+      else
+        st<S, R>(f + (size_t)(m * Q + n) * Np, i, (R)v, shift[m * Q + n]);
     }
 }
 
@@ -195,6 +198,7 @@ struct txg_handle {
   uint8_t* tag = nullptr;
   void* u_ic = nullptr;  // R[N*D*C]
   void* shift = nullptr; // R[C*Q] storage shifts for k_load_fi
+  void* f0[2] = {nullptr, nullptr};  // wide rest populations (FP32: double, FP16: float storage)
   // FP64 export arrays
   double* ex = nullptr;
   size_t ex_len = 0;
@@ -265,6 +269,8 @@ Bufs<S, R> make_bufs(txg_handle* h, int in) {
   b.rho = (R*)h->rho;
   b.mask = h->mask;
   b.tag = h->tag;
+  b.f0in = (const typename Rest<S>::type*)h->f0[in];
+  b.f0out = (typename Rest<S>::type*)h->f0[1 - in];
   return b;
 }
 
@@ -543,12 +549,18 @@ int txg_create(const txg_desc* d, txg_handle** out) {
                   h->alloc((void**)&h->tag, (size_t)h->Np) && h->alloc(&h->shift, (size_t)h->C * Q * R) &&
                   h->alloc((void**)&h->red, 4096 * sizeof(double)) &&
                   h->alloc((void**)&h->sums, (size_t)(kSumBlocks + 1) * h->ks.nsums * sizeof(double));
+  const size_t rest_bytes = (size_t)h->C * h->Np * (h->prec == TXG_FP32 ? 8 : 4);
+  if (alloc_ok && h->prec != TXG_FP64) alloc_ok = h->alloc(&h->f0[0], rest_bytes) && h->alloc(&h->f0[1], rest_bytes);
   if (!alloc_ok || hipStreamSynchronize(h->stream) != hipSuccess) {
     txg_destroy(h);
     return bad(TXG_ERR_NOMEM, "device allocation failed");
   }
   (void)hipMemsetAsync(h->f[0], 0, fbytes, h->stream);
   (void)hipMemsetAsync(h->f[1], 0, fbytes, h->stream);
+  if (h->f0[0]) {
+    (void)hipMemsetAsync(h->f0[0], 0, rest_bytes, h->stream);
+    (void)hipMemsetAsync(h->f0[1], 0, rest_bytes, h->stream);
+  }
   (void)hipMemsetAsync(h->mask, 0, (size_t)h->Np * 4, h->stream);
   (void)hipMemsetAsync(h->tag, 0, (size_t)h->Np, h->stream);
   {
@@ -675,14 +687,14 @@ int txg_set_fi(txg_handle* h, const double* fi) {
   if (!rc) {
     dim3 g(cdiv(h->N, 256));
     if (h->prec == TXG_FP64)
-      hipLaunchKernelGGL((k_load_fi<double, double>), g, dim3(256), 0, h->stream, (double*)h->f[0], tmp, h->N, h->Np,
-                         h->C, h->Q, (const double*)h->shift);
+      hipLaunchKernelGGL((k_load_fi<double, double>), g, dim3(256), 0, h->stream, (double*)h->f[0], (double*)nullptr,
+                         tmp, h->N, h->Np, h->C, h->Q, (const double*)h->shift);
     else if (h->prec == TXG_FP32)
-      hipLaunchKernelGGL((k_load_fi<float, float>), g, dim3(256), 0, h->stream, (float*)h->f[0], tmp, h->N, h->Np,
-                         h->C, h->Q, (const float*)h->shift);
+      hipLaunchKernelGGL((k_load_fi<float, float>), g, dim3(256), 0, h->stream, (float*)h->f[0], (double*)h->f0[0],
+                         tmp, h->N, h->Np, h->C, h->Q, (const float*)h->shift);
     else
-      hipLaunchKernelGGL((k_load_fi<__half, float>), g, dim3(256), 0, h->stream, (__half*)h->f[0], tmp, h->N, h->Np,
-                         h->C, h->Q, (const float*)h->shift);
+      hipLaunchKernelGGL((k_load_fi<__half, float>), g, dim3(256), 0, h->stream, (__half*)h->f[0], (float*)h->f0[0],
+                         tmp, h->N, h->Np, h->C, h->Q, (const float*)h->shift);
     rc = h->hip(hipStreamSynchronize(h->stream), "fi load");
   }
   h->release(tmp);
@@ -1083,6 +1095,8 @@ int txg_destroy(txg_handle* h) {
   if (h->stream) {
     h->release(h->f[0]);
     h->release(h->f[1]);
+    h->release(h->f0[0]);
+    h->release(h->f0[1]);
     h->release(h->rho);
     h->release(h->mask);
     h->release(h->tag);

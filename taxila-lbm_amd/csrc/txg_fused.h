@@ -73,6 +73,18 @@ __device__ __forceinline__ void bst(rsrc_t r, uint32_t off, R v, R shift) {
   }
 }
 
+// store an already quantized population
+template <class S>
+__device__ __forceinline__ void bst_raw(rsrc_t r, uint32_t off, S q) {
+  if constexpr (std::is_same<S, float>::value) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q), r, (int)off, 0, kStoreNT);
+  } else if constexpr (std::is_same<S, double>::value) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, q), r, (int)off, 0, kStoreNT);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(q), r, (int)off, 0, 0);
+  }
+}
+
 // byte offsets of the +-1 rows of a column: x, y*NX, z*NX*NY times sizeof(S)
 struct RowsB {
   uint32_t x[3], y[3], z[3];
@@ -81,12 +93,18 @@ struct RowsB {
 // pull-stream with bounce-back (DistributionStream + DistributionBounceback,
 // lbm_distribution_function.F90:560-784) through buffer resources
 template <int C, class S, class R>
-__device__ __forceinline__ void pull_buf(const Params<R>& p, const rsrc_t (&rin)[C], uint32_t mk, const RowsB& rb,
-                                         uint32_t plane_b, R (&f)[C][19]) {
+__device__ __forceinline__ void pull_buf(const Params<R>& p, const Bufs<S, R>& b, const rsrc_t (&rin)[C], uint32_t mk,
+                                         const RowsB& rb, uint32_t plane_b, R (&f)[C][19], R (&d0)[C]) {
   using L = Lat<3>;
   const uint32_t node_b = rb.x[1] + rb.y[1] + rb.z[1];
+#pragma unroll
+  for (int m = 0; m < C; m++) d0[m] = (R)0;
   sfor<0, 19>([&](auto NI) {
     constexpr int n = decltype(NI)::value;
+    if constexpr (n == 0 && wide_rest<S>()) {
+      load_rest<C>(b, (int)(node_b / (uint32_t)sizeof(S)), p.Np, f, d0);
+      return;
+    }
     uint32_t off;
     if constexpr (n == 0) {
       off = node_b;
@@ -105,16 +123,42 @@ struct BufStore {
   // holds copies, not a reference to the parameter block (see PtrStore)
   const rsrc_t* rout;
   uint32_t node_b, plane_b;
+  typename Rest<S>::type* __restrict__ f0out;
+  int Np;
   R sh[MAXC][3];
-  __device__ __forceinline__ BufStore(const Params<R>& p, const rsrc_t* r, uint32_t nb, uint32_t pb)
-      : rout(r), node_b(nb), plane_b(pb) {
+  R d0[MAXC];           // rest-population residuals of the loaded state
+  mutable double left;  // wide rest: mass of the component not yet stored
+  __device__ __forceinline__ BufStore(const Params<R>& p, const Bufs<S, R>& b, const rsrc_t* r, uint32_t nb,
+                                      uint32_t pb, const R (&res)[MAXC])
+      : rout(r), node_b(nb), plane_b(pb), f0out(b.f0out), Np(p.Np), left(0.0) {
 #pragma unroll
-    for (int m = 0; m < MAXC; m++)
+    for (int m = 0; m < MAXC; m++) {
 #pragma unroll
       for (int k = 0; k < 3; k++) sh[m][k] = p.shiftw[m][k];
+      d0[m] = res[m];
+    }
+  }
+  __device__ __forceinline__ void begin(int m, const R (&fm)[19]) const {
+    if constexpr (wide_rest<S>()) {
+      double s = (double)fm[0] + (double)d0[m];
+#pragma unroll
+      for (int n = 1; n < 19; n++) s += (double)fm[n];
+      left = s;
+    }
   }
   __device__ __forceinline__ void operator()(int m, int n, R v) const {
-    bst<S, R>(rout[m], node_b + (uint32_t)n * plane_b, v, sh[m][wclass<3>(n)]);
+    if constexpr (wide_rest<S>()) {
+      if (n == 0) {
+        f0out[(size_t)m * Np + node_b / (uint32_t)sizeof(S)] = (typename Rest<S>::type)left;
+        return;
+      }
+      const R sh_n = sh[m][wclass<3>(n)];
+      const S q = quantize<S, R>(v, sh_n);
+      left -= (double)dequantize<S, R>(q, sh_n);
+      bst_raw<S>(rout[m], node_b + (uint32_t)n * plane_b, q);
+    } else {
+      bst<S, R>(rout[m], node_b + (uint32_t)n * plane_b, v, sh[m][wclass<3>(n)]);
+    }
   }
 };
 
@@ -169,9 +213,6 @@ __device__ __forceinline__ void ring_col(int q, int& cx, int& cy) {
   cx = k < RR ? k : TX + k;
 }
 
-#ifndef TXG_FUSED_WAVES_PER_EU
-#define TXG_FUSED_WAVES_PER_EU 1
-#endif
 
 // stencil geometry of the one-pass kernel for isotropy stencil NFF
 template <int NFF, int TX, int TY>
@@ -180,6 +221,10 @@ struct FusedShape {
   static constexpr int EXW = TX + 2 * RR, EYW = TY + 2 * RR, NCOL = EXW * EYW, NPL = 2 * RR + 1;
   template <int C>
   static constexpr int lds_len() { return RR > 0 ? NPL * C * NCOL : 1; }
+  // FP32 arithmetic: keep two waves per SIMD (the register budget is
+  // 256 VGPRs); FP64 needs more registers than two waves allow
+  template <class R>
+  static constexpr int waves() { return std::is_same<R, float>::value ? 2 : 1; }
 };
 
 // one work item of the one-pass step: the TX x TY column tile at (x0, y0)
@@ -284,7 +329,7 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
   // pull the column at rows (xr, yr, zr) with link mask mk; density
   // (Dirichlet-overridden) to rho
   auto pull_col = [&](const uint32_t (&xr)[3], const uint32_t (&yr)[3], const uint32_t (&zr)[3], uint32_t mk,
-                      R (&f)[C][Q], R (&rho)[C]) {
+                      R (&f)[C][Q], R (&d0)[C], R (&rho)[C]) {
     RowsB cr;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -295,7 +340,7 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
 #pragma unroll
     for (int m = 0; m < C; m++) rho[m] = (R)NAN;
     if (mk & kSelfSolid) return;
-    pull_buf<C, S, R>(p, rin, mk, cr, plane_b, f);
+    pull_buf<C, S, R>(p, b, rin, mk, cr, plane_b, f, d0);
 #pragma unroll
     for (int m = 0; m < C; m++) {
       R s = (R)0;
@@ -327,32 +372,32 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
 
   // phase A for plane zz: own column into f (kept for phase B), ring
   // columns; mo and mr are the plane's masks (loaded one step earlier)
-  auto phase_a = [&](int zz, R (&f)[C][Q], uint32_t mo, const RingMasks& mr) {
+  auto phase_a = [&](int zz, R (&f)[C][Q], R (&d0)[C], uint32_t mo, const RingMasks& mr) {
     const int slot = slot_of(zz);
     uint32_t zr[3];
     zrows(zz, zr);
     R rho[C];
-    pull_col(ox, oy, zr, mo, f, rho);
+    pull_col(ox, oy, zr, mo, f, d0, rho);
 #pragma unroll
     for (int m = 0; m < C; m++) srho[(slot * C + m) * NCOL + ocol] = rho[m];
 #pragma unroll
     for (int k = 0; k < NRP; k++) {
       if (rcol[k] < 0) continue;
-      R rr[C];
+      R rr[C], dr[C];
       R fr[C][Q];
-      pull_col(rx[k], ry[k], zr, mr.m[k], fr, rr);
+      pull_col(rx[k], ry[k], zr, mr.m[k], fr, dr, rr);
 #pragma unroll
       for (int m = 0; m < C; m++) srho[(slot * C + m) * NCOL + rcol[k]] = rr[m];
     }
   };
-  auto phase_a_now = [&](int zz, R (&f)[C][Q]) -> uint32_t {
+  auto phase_a_now = [&](int zz, R (&f)[C][Q], R (&d0)[C]) -> uint32_t {
     const uint32_t mo = own_mask(zz);
-    phase_a(zz, f, mo, ring_masks(zz));
+    phase_a(zz, f, d0, mo, ring_masks(zz));
     return mo;
   };
 
   // phase B for plane z with the own column's populations f
-  auto phase_b = [&](int z, R (&f)[C][Q], uint32_t mk) {
+  auto phase_b = [&](int z, R (&f)[C][Q], const R (&d0)[C], uint32_t mk) {
     if (!active) return;
     set_z<3, 1, R>(p, z, nb);
     const int node = nb.xw[1] + nb.yw[1] + nb.zw[1];
@@ -368,7 +413,9 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
     for (int m = 0; m < C; m++) rs[m] = acc.template get<0, 0, 0>(m);
     R rl[C], j[C][3], F[C][3], up[3];
     moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
-    collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, rout, (uint32_t)node * SB, plane_b), f, rl, F, up);
+    R res[MAXC];
+    pad_res<C, R>(d0, res);
+    collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, b, rout, (uint32_t)node * SB, plane_b, res), f, rl, F, up);
   };
 
   if constexpr (RR == 0) {
@@ -386,8 +433,8 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
         cr.y[d] = oy[d];
         cr.z[d] = (uint32_t)nb.zw[d] * SB;
       }
-      R f[C][Q];
-      pull_buf<C, S, R>(p, rin, mk, cr, plane_b, f);
+      R f[C][Q], d0[C];
+      pull_buf<C, S, R>(p, b, rin, mk, cr, plane_b, f, d0);
       R rs[C];
 #pragma unroll
       for (int m = 0; m < C; m++) {
@@ -410,14 +457,16 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
       acc.slot[0] = 0;
       R rl[C], j[C][3], F[C][3], up[3];
       moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
-      collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, rout, (uint32_t)node * SB, plane_b), f, rl, F, up);
+      R res[MAXC];
+      pad_res<C, R>(d0, res);
+      collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, b, rout, (uint32_t)node * SB, plane_b, res), f, rl, F, up);
     }
   } else if constexpr (!KEEP) {
     // registers too few for a ring of RR+1 planes: phase B re-pulls plane z
     // (read RR planes earlier by this workgroup, so mostly from L2)
     const int zs = dir > 0 ? z0 : z1 - 1, nzs = z1 - z0;
-    R f[C][Q];
-    for (int k = -RR; k < RR; k++) phase_a_now(zs + k * dir, f);
+    R f[C][Q], d0[C];
+    for (int k = -RR; k < RR; k++) phase_a_now(zs + k * dir, f, d0);
     uint32_t mo = own_mask(zs + RR * dir);
     RingMasks mr = ring_masks(zs + RR * dir);
     for (int i = 0; i < nzs; i++) {
@@ -425,7 +474,7 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
       const uint32_t mo2 = own_mask(z + (RR + 1) * dir);
       const RingMasks mr2 = ring_masks(z + (RR + 1) * dir);
       const uint32_t mz = own_mask(z);
-      phase_a(z + RR * dir, f, mo, mr);
+      phase_a(z + RR * dir, f, d0, mo, mr);
       __syncthreads();
       if (active && !(mz & kSelfSolid)) {
         uint32_t zr[3];
@@ -437,9 +486,9 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
           cr.y[d] = oy[d];
           cr.z[d] = zr[d];
         }
-        pull_buf<C, S, R>(p, rin, mz, cr, plane_b, f);
+        pull_buf<C, S, R>(p, b, rin, mz, cr, plane_b, f, d0);
       }
-      phase_b(z, f, mz);
+      phase_b(z, f, d0, mz);
       __syncthreads();
       mo = mo2;
       mr = mr2;
@@ -449,56 +498,58 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
     // in phase A are the ones phase B collides one plane later; the masks of
     // the plane after are loaded one step before their phase A
     const int zs = dir > 0 ? z0 : z1 - 1, nzs = z1 - z0;
-    R fa[C][Q], fb[C][Q];
-    phase_a_now(zs - dir, fb);  // density only
-    uint32_t ma = phase_a_now(zs, fa), mb = 0;
+    R fa[C][Q], fb[C][Q], da[C], db[C];
+    phase_a_now(zs - dir, fb, db);  // density only
+    uint32_t ma = phase_a_now(zs, fa, da), mb = 0;
     uint32_t mo = own_mask(zs + dir);
     RingMasks mr = ring_masks(zs + dir);
-    auto step = [&](int z, R (&cur)[C][Q], uint32_t& mcur, R (&nxt)[C][Q], uint32_t& mnxt) {
+    auto step = [&](int z, R (&cur)[C][Q], R (&dcur)[C], uint32_t& mcur, R (&nxt)[C][Q], R (&dnxt)[C],
+                    uint32_t& mnxt) {
       const uint32_t mo2 = own_mask(z + 2 * dir);
       const RingMasks mr2 = ring_masks(z + 2 * dir);
-      phase_a(z + dir, nxt, mo, mr);
+      phase_a(z + dir, nxt, dnxt, mo, mr);
       mnxt = mo;
       __syncthreads();
-      phase_b(z, cur, mcur);
+      phase_b(z, cur, dcur, mcur);
       __syncthreads();
       mo = mo2;
       mr = mr2;
     };
     for (int i = 0; i < nzs; i += 2) {
-      step(zs + i * dir, fa, ma, fb, mb);
-      if (i + 1 < nzs) step(zs + (i + 1) * dir, fb, mb, fa, ma);
+      step(zs + i * dir, fa, da, ma, fb, db, mb);
+      if (i + 1 < nzs) step(zs + (i + 1) * dir, fb, db, mb, fa, da, ma);
     }
   } else {
     // register ring of three planes
     const int zs = dir > 0 ? z0 : z1 - 1, nzs = z1 - z0;
-    R f0[C][Q], f1[C][Q], f2[C][Q];
-    phase_a_now(zs - 2 * dir, f2);  // density only
-    phase_a_now(zs - dir, f2);      // density only
-    uint32_t m0 = phase_a_now(zs, f0), m1 = phase_a_now(zs + dir, f1), m2 = 0;
+    R f0[C][Q], f1[C][Q], f2[C][Q], d0[C], d1[C], d2[C];
+    phase_a_now(zs - 2 * dir, f2, d2);  // density only
+    phase_a_now(zs - dir, f2, d2);      // density only
+    uint32_t m0 = phase_a_now(zs, f0, d0), m1 = phase_a_now(zs + dir, f1, d1), m2 = 0;
     uint32_t mo = own_mask(zs + 2 * dir);
     RingMasks mr = ring_masks(zs + 2 * dir);
-    auto step = [&](int z, R (&cur)[C][Q], uint32_t& mcur, R (&nxt)[C][Q], uint32_t& mnxt) {
+    auto step = [&](int z, R (&cur)[C][Q], R (&dcur)[C], uint32_t& mcur, R (&nxt)[C][Q], R (&dnxt)[C],
+                    uint32_t& mnxt) {
       const uint32_t mo2 = own_mask(z + 3 * dir);
       const RingMasks mr2 = ring_masks(z + 3 * dir);
-      phase_a(z + 2 * dir, nxt, mo, mr);
+      phase_a(z + 2 * dir, nxt, dnxt, mo, mr);
       mnxt = mo;
       __syncthreads();
-      phase_b(z, cur, mcur);
+      phase_b(z, cur, dcur, mcur);
       __syncthreads();
       mo = mo2;
       mr = mr2;
     };
     for (int i = 0; i < nzs; i += 3) {
-      step(zs + i * dir, f0, m0, f2, m2);
-      if (i + 1 < nzs) step(zs + (i + 1) * dir, f1, m1, f0, m0);
-      if (i + 2 < nzs) step(zs + (i + 2) * dir, f2, m2, f1, m1);
+      step(zs + i * dir, f0, d0, m0, f2, d2, m2);
+      if (i + 1 < nzs) step(zs + (i + 1) * dir, f1, d1, m1, f0, d0, m0);
+      if (i + 2 < nzs) step(zs + (i + 2) * dir, f2, d2, m2, f1, d1, m1);
     }
   }
 }
 
 template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(TXG_FUSED_WAVES_PER_EU)))
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(FusedShape<NFF, TX, TY>::template waves<R>())))
 k_fused3(Params<R> p, Bufs<S, R> b, FusedGeom g) {
   __shared__ R srho[FusedShape<NFF, TX, TY>::template lds_len<C>()];
   // segments of one tile are consecutive work items (same XCD, so the
@@ -529,7 +580,7 @@ struct BatchMember {
 // XCD's L2 serves contiguous tiles of the same member.  flip toggles every
 // member's orientation (every step flips all of them).
 template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(TXG_FUSED_WAVES_PER_EU)))
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(FusedShape<NFF, TX, TY>::template waves<R>())))
 k_fused3_batch(const BatchMember<S, R>* __restrict__ mem, const int* __restrict__ prefix, int nmem, int flip) {
   __shared__ R srho[FusedShape<NFF, TX, TY>::template lds_len<C>()];
   constexpr int NX = 8;

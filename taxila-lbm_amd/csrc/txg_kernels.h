@@ -84,6 +84,27 @@ struct Params {
   R ic15[MAXC];        // 1/(2 c_s2)
 };
 
+// Rest population f_0 of the reduced-precision storage modes: stored one
+// format wider (FP32 populations: double; FP16: float) in an array of its
+// own, and written as (mass that entered the node) - (the other stored
+// populations, as the next step will read them).  The sums are formed in
+// FP64, so each node hands on the mass it received up to one rounding of
+// f_0 in the wide format: the storage and the FP32 collision do not drift
+// the total mass.  FP64 storage keeps f_0 in the population array (the
+// reference's arithmetic, bit for bit).
+template <class S>
+struct Rest {
+  using type = double;  // unused with FP64 storage
+};
+template <>
+struct Rest<__half> {
+  using type = float;
+};
+template <class S>
+constexpr bool wide_rest() {
+  return !std::is_same<S, double>::value;
+}
+
 template <class S, class R>
 struct Bufs {
   const S* __restrict__ fin;
@@ -91,6 +112,8 @@ struct Bufs {
   R* __restrict__ rho;
   const uint32_t* __restrict__ mask;
   const uint8_t* __restrict__ tag;
+  const typename Rest<S>::type* __restrict__ f0in;  // wide rest populations [m*Np + node]
+  typename Rest<S>::type* __restrict__ f0out;
 };
 
 // ------------------------------------------------------------ storage
@@ -109,6 +132,29 @@ __device__ __forceinline__ void st(S* p, int i, R v, R shift) {
     p[i] = __float2half_rn((float)(v - shift));
   else
     __builtin_nontemporal_store((S)v, p + i);
+}
+
+// stored form of v and the value a later load makes of it
+template <class S, class R>
+__device__ __forceinline__ S quantize(R v, R shift) {
+  if constexpr (std::is_same<S, __half>::value)
+    return __float2half_rn((float)(v - shift));
+  else
+    return (S)v;
+}
+template <class S, class R>
+__device__ __forceinline__ R dequantize(S q, R shift) {
+  if constexpr (std::is_same<S, __half>::value)
+    return (R)__half2float(q) + shift;
+  else
+    return (R)q;
+}
+template <class S>
+__device__ __forceinline__ void st_raw(S* p, int i, S q) {
+  if constexpr (std::is_same<S, __half>::value)
+    p[i] = q;
+  else
+    __builtin_nontemporal_store(q, p + i);
 }
 
 // weight class of direction n: 0 rest, 1 axis, 2 diagonal (D3Q19 and D2Q9
@@ -181,14 +227,33 @@ __device__ __forceinline__ int nidx(const NbrH<H>& nb) {
 }
 
 // ------------------------------------------------------------ pull-stream
+// wide rest population of node into f[m][0] (rounded to R) and the
+// rounding residual d0[m] (zero when R is as wide as the stored value)
+template <int C, class S, class R, int Q>
+__device__ __forceinline__ void load_rest(const Bufs<S, R>& b, int node, int Np, R (&f)[C][Q], R (&d0)[C]) {
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    const auto w = b.f0in[(size_t)m * Np + node];
+    f[m][0] = (R)w;
+    d0[m] = (R)((double)w - (double)f[m][0]);
+  }
+}
+
 // f_n(x) = f*_n(x - c_n), or f*_opp(n)(x) when x - c_n is a wall
 // (DistributionStream + DistributionBounceback, lbm_distribution_function.F90:560-784)
 template <int D, int C, class S, class R, int H>
-__device__ __forceinline__ void pull(const Params<R>& p, const S* __restrict__ fin, int node,
-                                     uint32_t mk, const NbrH<H>& nb, R (&f)[C][Lat<D>::Q]) {
+__device__ __forceinline__ void pull(const Params<R>& p, const Bufs<S, R>& b, int node, uint32_t mk,
+                                     const NbrH<H>& nb, R (&f)[C][Lat<D>::Q], R (&d0)[C]) {
   using L = Lat<D>;
+  const S* __restrict__ fin = b.fin;
+#pragma unroll
+  for (int m = 0; m < C; m++) d0[m] = (R)0;
   sfor<0, L::Q>([&](auto NI) {
     constexpr int n = decltype(NI)::value;
+    if constexpr (n == 0 && wide_rest<S>()) {
+      load_rest<C>(b, node, p.Np, f, d0);
+      return;
+    }
     int src;
     int dir;
     if (n > 0 && (mk & (1u << n))) {
@@ -207,13 +272,31 @@ __device__ __forceinline__ void pull(const Params<R>& p, const S* __restrict__ f
 
 // read the populations of node without streaming (initial state)
 template <int D, int C, class S, class R>
-__device__ __forceinline__ void load_local(const Params<R>& p, const S* __restrict__ fin, int node,
-                                           R (&f)[C][Lat<D>::Q]) {
+__device__ __forceinline__ void load_local(const Params<R>& p, const Bufs<S, R>& b, int node, R (&f)[C][Lat<D>::Q],
+                                           R (&d0)[C]) {
   using L = Lat<D>;
 #pragma unroll
-  for (int n = 0; n < L::Q; n++)
+  for (int m = 0; m < C; m++) d0[m] = (R)0;
 #pragma unroll
-    for (int m = 0; m < C; m++) f[m][n] = ld<S, R>(fin + (size_t)(m * L::Q + n) * p.Np, node, storage_shift<D>(p, m, n));
+  for (int n = wide_rest<S>() ? 1 : 0; n < L::Q; n++)
+#pragma unroll
+    for (int m = 0; m < C; m++)
+      f[m][n] = ld<S, R>(b.fin + (size_t)(m * L::Q + n) * p.Np, node, storage_shift<D>(p, m, n));
+  if constexpr (wide_rest<S>()) load_rest<C>(b, node, p.Np, f, d0);
+}
+
+// the node's incoming mass in FP64: the rest population at its stored
+// precision (f_0 + d0) plus the other populations as they entered the
+// collision
+template <int C, int Q, class R>
+__device__ __forceinline__ void incoming_mass(const R (&f)[C][Q], const R (&d0)[C], double (&mass)[C]) {
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    double s = (double)f[m][0] + (double)d0[m];
+#pragma unroll
+    for (int n = 1; n < Q; n++) s += (double)f[m][n];
+    mass[m] = s;
+  }
 }
 
 // ------------------------------------------------------------ Dirichlet BC
@@ -521,15 +604,15 @@ __global__ void __launch_bounds__(256) k_density(Params<R> p, Bufs<S, R> b, int 
   if (node >= p.N) return;
   const uint32_t mk = b.mask[node];
   if (mk & kSelfSolid) return;
-  R f[C][L::Q];
+  R f[C][L::Q], d0[C];
   if (pulled) {
     int x, y, z;
     node_xyz(p, node, x, y, z);
     Nbr nb;
     make_nbr<D>(p, x, y, z, nb);
-    pull<D, C, S, R>(p, b.fin, node, mk, nb, f);
+    pull<D, C, S, R>(p, b, node, mk, nb, f, d0);
   } else {
-    load_local<D, C, S, R>(p, b.fin, node, f);
+    load_local<D, C, S, R>(p, b, node, f, d0);
   }
   R rho[C];
 #pragma unroll
@@ -634,6 +717,7 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, const ST& stor
   for (int d = 0; d < D; d++) usqr = usqr + up[d] * up[d];
 #pragma unroll
   for (int m = 0; m < C; m++) {
+    store.begin(m, f[m]);  // the component's populations before the collision
     const R r = rl[m];
     const R rcs = r * p.cs2[m];
     R feq[Q], pref[Q];
@@ -740,8 +824,11 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, const ST& stor
 #pragma unroll
       for (int n = 0; n < Q; n++) f[m][n] = f[m][n] + pref[n] * feq[n];
     }
+    // the rest population last: a mass-conserving store forms it from the
+    // others as stored
 #pragma unroll
-    for (int n = 0; n < Q; n++) store(m, n, f[m][n]);
+    for (int n = 1; n < Q; n++) store(m, n, f[m][n]);
+    store(m, 0, f[m][0]);
   }
 }
 
@@ -751,19 +838,51 @@ struct PtrStore {
   // by value: a reference to the kernel's parameter block would make the
   // compiler copy the whole block to scratch memory
   S* __restrict__ fout;
+  typename Rest<S>::type* __restrict__ f0out;
   int node;
   int Np;
   R sh[MAXC][3];
-  __device__ __forceinline__ PtrStore(const Params<R>& p, S* o, int nd) : fout(o), node(nd), Np(p.Np) {
+  R d0[MAXC];           // rest-population residuals of the loaded state
+  mutable double left;  // wide rest: mass of the component not yet stored
+  __device__ __forceinline__ PtrStore(const Params<R>& p, const Bufs<S, R>& b, int nd, const R (&res)[MAXC])
+      : fout(b.fout), f0out(b.f0out), node(nd), Np(p.Np), left(0.0) {
 #pragma unroll
-    for (int m = 0; m < MAXC; m++)
+    for (int m = 0; m < MAXC; m++) {
 #pragma unroll
       for (int k = 0; k < 3; k++) sh[m][k] = p.shiftw[m][k];
+      d0[m] = res[m];
+    }
+  }
+  // the component's incoming mass in FP64 (rest population at its stored width)
+  __device__ __forceinline__ void begin(int m, const R (&fm)[Lat<D>::Q]) const {
+    if constexpr (wide_rest<S>()) {
+      double s = (double)fm[0] + (double)d0[m];
+#pragma unroll
+      for (int n = 1; n < Lat<D>::Q; n++) s += (double)fm[n];
+      left = s;
+    }
   }
   __device__ __forceinline__ void operator()(int m, int n, R v) const {
-    st<S, R>(fout + (size_t)(m * Lat<D>::Q + n) * Np, node, v, sh[m][wclass<D>(n)]);
+    if constexpr (wide_rest<S>()) {
+      if (n == 0) {
+        f0out[(size_t)m * Np + node] = (typename Rest<S>::type)left;
+        return;
+      }
+      const S q = quantize<S, R>(v, sh[m][wclass<D>(n)]);
+      left -= (double)dequantize<S, R>(q, sh[m][wclass<D>(n)]);
+      st_raw<S>(fout + (size_t)(m * Lat<D>::Q + n) * Np, node, q);
+    } else {
+      st<S, R>(fout + (size_t)(m * Lat<D>::Q + n) * Np, node, v, sh[m][wclass<D>(n)]);
+    }
   }
 };
+
+// residuals of C components padded to MAXC
+template <int C, class R>
+__device__ __forceinline__ void pad_res(const R (&d0)[C], R (&out)[MAXC]) {
+#pragma unroll
+  for (int m = 0; m < MAXC; m++) out[m] = m < C ? d0[m] : (R)0;
+}
 
 template <int D, int C, int NFF, bool FAST, class S, class R>
 __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int pulled, int apply_bc,
@@ -795,11 +914,11 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
   node_xyz(p, node, x, y, z);
   Nbr nb;
   make_nbr<D>(p, x, y, z, nb);
-  R f[C][Q];
+  R f[C][Q], d0[C];
   if (pulled)
-    pull<D, C, S, R>(p, b.fin, node, mk, nb, f);
+    pull<D, C, S, R>(p, b, node, mk, nb, f, d0);
   else
-    load_local<D, C, S, R>(p, b.fin, node, f);
+    load_local<D, C, S, R>(p, b, node, f, d0);
   if (apply_bc && (mk & (0x3fu << kFaceShift))) apply_dirichlet<D, C, R>(p, mk, f);
   R rs[C];
 #pragma unroll
@@ -812,6 +931,7 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
     for (int m = 0; m < C; m++) {
 #pragma unroll
       for (int n = 0; n < Q; n++) ex.f[(size_t)(m * Q + n) * p.N + node] = (double)f[m][n];
+      ex.f[(size_t)(m * Q) * p.N + node] = (double)f[m][0] + (double)d0[m];  // the rest at its stored width
       ex.rho[(size_t)m * p.N + node] = (double)rl[m];
 #pragma unroll
       for (int d = 0; d < D; d++) {
@@ -823,7 +943,9 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
     for (int d = 0; d < D; d++) ex.up[(size_t)d * p.N + node] = (double)up[d];
     return;
   }
-  collide_store<D, C, FAST, R>(p, PtrStore<D, S, R>(p, b.fout, node), f, rl, F, up);
+  R res[MAXC];
+  pad_res<C, R>(d0, res);
+  collide_store<D, C, FAST, R>(p, PtrStore<D, S, R>(p, b, node, res), f, rl, F, up);
 }
 
 // ------------------------------------------------------------ flow sums
@@ -856,11 +978,11 @@ __global__ void __launch_bounds__(256) k_sums(Params<R> p, Bufs<S, R> b, int pul
     node_xyz(p, node, x, y, z);
     Nbr nb;
     make_nbr<D>(p, x, y, z, nb);
-    R f[C][L::Q];
+    R f[C][L::Q], d0[C];
     if (pulled)
-      pull<D, C, S, R>(p, b.fin, node, mk, nb, f);
+      pull<D, C, S, R>(p, b, node, mk, nb, f, d0);
     else
-      load_local<D, C, S, R>(p, b.fin, node, f);
+      load_local<D, C, S, R>(p, b, node, f, d0);
     R rs[C];
     if constexpr (NFF == 0) {
       // no stencil: the node's own density (Dirichlet faces prescribe it),
@@ -888,9 +1010,12 @@ __global__ void __launch_bounds__(256) k_sums(Params<R> p, Bufs<S, R> b, int pul
     R rl[C], j[C][3], F[C][3], up[3];
     moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
     double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
+    double mass[C];
+    incoming_mass<C, L::Q, R>(f, d0, mass);
 #pragma unroll
     for (int m = 0; m < C; m++) {
-      v[m] += (double)rl[m];
+      // the stored mass (the node's density after Dirichlet faces reset it)
+      v[m] += wide_rest<S>() ? mass[m] : (double)rl[m];
       rt += (double)p.mm[m] * (double)rl[m];
 #pragma unroll
       for (int d = 0; d < D; d++) {
@@ -934,6 +1059,9 @@ __global__ void __launch_bounds__(256) k_fiinit(Params<R> p, Bufs<S, R> b, const
     for (int m = 0; m < C; m++)
 #pragma unroll
       for (int n = 0; n < Q; n++) st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, (R)0, (R)0);
+    if constexpr (wide_rest<S>())
+#pragma unroll
+      for (int m = 0; m < C; m++) b.f0out[(size_t)m * p.Np + node] = 0;
     return;
   }
   int x, y, z;
@@ -990,7 +1118,11 @@ __global__ void __launch_bounds__(256) k_fiinit(Params<R> p, Bufs<S, R> b, const
         });
         feq = (R)L::w[n] * r * (p.a15[m] + ue / p.cs2[m] + ue * ue / p.two_cs2_sq[m] - usqr / p.two_cs2[m]);
       }
-      st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, ((R)1. - (R)0.5 * pref) * feq, storage_shift<D>(p, m, n));
+      const R fi = ((R)1. - (R)0.5 * pref) * feq;
+      if constexpr (n == 0 && wide_rest<S>())
+        b.f0out[(size_t)m * p.Np + node] = (typename Rest<S>::type)fi;
+      else
+        st<S, R>(b.fout + (size_t)(m * Q + n) * p.Np, node, fi, storage_shift<D>(p, m, n));
     });
   }
 }
