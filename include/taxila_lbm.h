@@ -50,9 +50,30 @@ extern "C" {
 #define TXG_FP32 1        /* FP32 compute, FP32 population storage */
 #define TXG_FP32_FP16 2   /* FP32 compute, shifted FP16 storage (f_i - w_i rho0) */
 
-/* per-face boundary kinds (include/lbm_definitions.h BC_* values) */
-#define TXG_BC_NULL 0     /* periodic face (if periodic[d]) or ghost-wall bounce-back */
-#define TXG_BC_DENSITY 6  /* Dirichlet density, -bc_density_<face> (lbm_flow.F90:1066-1093) */
+/* per-face boundary conditions: the reference's flow%bc_flags codes
+ * (include/lbm_definitions.h:30-43) as FlowParseBC sets them from the
+ * -bc_<kind>_<face> options (lbm_flow.F90:1031-1309).  The values of a
+ * face come from bc_density / bc_data below and are turned into per-node
+ * face values as FlowSetUpBCsD3/D2 does (lbm_flow.F90:1311-1956).  Nodes of
+ * a face get the face's node condition after bounce-back
+ * (BCApply, lbm_bc.F90:781-807): Dirichlet BCApplyDirichletNode
+ * (:1273-1333), Neumann BCApplyNeumannNode (:1533-1593), velocity
+ * BCApplyVelocityNode (:1793-1865), reflecting BCApplyReflectingD3/D2
+ * (:825-1088). */
+#define TXG_BC_NULL 0            /* periodic face (if periodic[d]) or ghost-wall bounce-back */
+#define TXG_BC_REFLECTING 2      /* -bc_reflecting_<face>; not on xm in 2-D (TXG_ERR_UNSUPPORTED) */
+#define TXG_BC_VELOCITY 5        /* -bc_velocity_<face>_values: bc_data[f][0..D) = u */
+#define TXG_BC_DENSITY 6         /* -bc_density_<face>[_<name>]: bc_density[f][m] */
+#define TXG_BC_MOMENTUM 7        /* -bc_momentum_<face>[_<name>]: bc_data[f][m*D + d] = (rho u)_{m,d} */
+#define TXG_BC_PRESSURE 8        /* -bc_pressure_<face>_value (1 component): bc_data[f][0] = p, rho = 3 p */
+#define TXG_BC_PRESSURE_INLET 9  /* -bc_pressure_inlet_<face>: bc_data[f][0] = p, [1] = rho_1/rho_t (2 components) */
+#define TXG_BC_PRESSURE_OUTLET 10 /* -bc_pressure_outlet_<face>: bc_data[f][0]; 1 component only here */
+#define TXG_BC_FLUX 11           /* -bc_flux_<face>_value (1 component): inward-normal flux bc_data[f][0] */
+#define TXG_BC_FLUX_INLET 12     /* -bc_flux_inlet_<face>: bc_data[f][0] = flux, [1] = rho_1/rho_t (2 components) */
+#define TXG_BC_FLUX_OUTLET 13    /* -bc_flux_outlet_<face>_value: bc_data[f][0] */
+
+/* values per face in bc_data (the reference's flow%bc_data(:, boundary)) */
+#define TXG_MAX_BCDATA (3 * TXG_MAX_COMPONENTS)
 
 /* faces, in the reference order BOUNDARY_XM..BOUNDARY_ZP (1..6) minus one */
 #define TXG_XM 0
@@ -96,6 +117,7 @@ typedef struct txg_desc {
   double gw[TXG_MAX_MINERALS][TXG_MAX_COMPONENTS];  /* gw[mineral-1][m] = -gw_<mineral>_<name> */
   double bc_density[6][TXG_MAX_COMPONENTS];          /* -bc_density_<face>_<name> */
   double rho0[TXG_MAX_COMPONENTS];  /* FP16 storage shift density per component */
+  double bc_data[6][TXG_MAX_BCDATA];  /* option values of the face's BC (see TXG_BC_*) */
 } txg_desc;
 
 typedef struct txg_handle txg_handle;
@@ -119,6 +141,17 @@ int txg_set_walls(txg_handle* h, const double* walls);
 /* Replaces the initialize_state callback's output
  * (default_initialize_state.F90:19): initial rho and u on owned nodes. */
 int txg_set_state(txg_handle* h, const double* rho, const double* u);
+
+/* Replaces the initialize_bcs callback of BCSetValues (main.F90:64,
+ * lbm_bc.F90:215-228): sets the per-node values of face f (TXG_XM..TXG_ZP),
+ * which must carry a boundary condition, to vals[(i*D + d)*C + m] -- the
+ * reference's xm_vals(m, d, j, k) -- for the face's nodes i in natural
+ * order of the two in-face axes (xm/xp: y fastest, then z; ym/yp: x, then
+ * z; zm/zp: x, then y).  Dirichlet faces read vals(m, 1) as the density of
+ * component m, Neumann faces vals(m, d) as the momentum, velocity faces
+ * vals(1, d) as the velocity.  Takes effect at the next step; the values
+ * from the descriptor (FlowSetUpBCs) are the default. */
+int txg_set_bc_values(txg_handle* h, int32_t face, const double* vals);
 
 /* Restart / IC-from-file path (lbm.F90:482-544): load populations directly;
  * txg_init then skips FlowFiInit, as LBMInit2 does (lbm.F90:209-213). */
@@ -201,7 +234,8 @@ int txg_tune_info(const txg_handle* h, int32_t* tile_x, int32_t* tile_y, int32_t
  * bit 19 when a neighbour x + c_n is a mineral (a wettability link,
  * LBMAddFluidSolidForcesD3, lbm_forcing.F90:1326-1373); bit 20 when no wall
  * lies within the psi-gradient stencil's reach; bits 24-29 when the node is
- * on a Dirichlet face xm..zp; bit 31 alone on wall nodes.  Cells beyond a
+ * on face xm..zp and that face has a boundary condition; bit 31 alone on
+ * wall nodes.  Cells beyond a
  * non-periodic face are walls (ghost tag 999, lbm_walls.F90:208-231). */
 int txg_get_mask(txg_handle* h, uint32_t* mask);
 

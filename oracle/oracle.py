@@ -48,6 +48,7 @@ def lib():
         L.orc_delta_norm.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_phase.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_bc_values.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_ff_table.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 5 + [C.c_int]
         _lib = L
     return _lib
@@ -90,12 +91,19 @@ class Oracle:
         r = np.ascontiguousarray(rho, dtype=np.float64).ravel()
         assert r.size == self.N * self.C
         uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64).ravel()
+        assert uu is None or uu.size == self.N * self.D * self.C, "u is [(node*D + d)*C + m]"
         lib().orc_set_state(self.h, _p(r), _p(uu))
 
     def set_fi(self, fi):
         f = np.ascontiguousarray(fi, dtype=np.float64).ravel()
         assert f.size == self.N * self.Q * self.C
         lib().orc_set_fi(self.h, _p(f))
+
+    def set_bc_values(self, face, vals):
+        """Per-node values vals[(i*D + d)*C + m] of a face (BCSetValues)."""
+        v = np.ascontiguousarray(vals, dtype=np.float64).ravel()
+        if lib().orc_set_bc_values(self.h, int(face), _p(v)):
+            raise ValueError("oracle: face %d has no boundary values" % face)
 
     def init(self):
         lib().orc_init(self.h)

@@ -7,7 +7,10 @@
  * Scope: D2Q9 / D3Q19, SRT / MRT, fluid-fluid forces at isotropy 4/8 (and
  * 10 on D2Q9), per-mineral fluid-solid forces, body forces, bounce-back
  * (including ghost-999 faces and 900-902 specular tags), periodic faces and
- * Dirichlet density faces.  psi = rho (no -flow_use_nonideal_eos).
+ * the face boundary conditions: Dirichlet (density, pressure, pressure
+ * inlet, one-component pressure outlet), Neumann (momentum, flux, flux
+ * inlet/outlet), velocity and reflecting, with per-node face values.
+ * psi = rho (no -flow_use_nonideal_eos).
  *
  * Arrays follow the reference's local (ghosted) PETSc arrays, 1-based owned
  * coordinates, component index fastest:
@@ -21,6 +24,11 @@
  * Parity status of each part (how the restatement is pinned):
  *   - D2Q9 SRT iso-4 and D2Q9 MRT iso-10, periodic: the reference's golden
  *     fixtures tests/bubble_2D{,_hots}/reference_solution/fi001.dat.
+ *   - Neumann, velocity and reflecting faces: the node conditions they
+ *     impose (j + F/2 equal to the prescribed momentum; j + F/2 = rho u
+ *     with the corrected density; mirrored links), the steady flux through
+ *     every cross-section of a flux- or velocity-driven channel and the
+ *     Poiseuille profile of a pressure-driven one.
  *   - D3Q19 equilibrium, MRT, iso-8 stencil, fluid-solid and body forces,
  *     bounce-back, Dirichlet faces: closed-form and invariant checks
  *     (moments of feq, MRT with equal rates == SRT, mass/momentum
@@ -71,7 +79,12 @@ struct orc {
   /* components (lbm_component.F90:131-176, lbm_relaxation.F90:117-151) */
   double d_k[MAXC], c_s2[MAXC], tau[MAXC], s_c[MAXC], tau_mrt[MAXC][19];
   int ff, fs, body;
-  int dirichlet[6];
+  /* boundary conditions: type per face (BC_REFLECTING 2, BC_DIRICHLET 3,
+   * BC_NEUMANN 4, BC_VELOCITY 5, or 0) and the per-node face values
+   * vals(1:C, 1:D, face node) of FlowSetUpBCs / BCSetValues */
+  int bckind[6];
+  double* bcv[6];
+  long bcn[6];
   int fi_loaded, initialized;
   /* fluid-fluid stencil */
   int nff;
@@ -364,6 +377,142 @@ int orc_ff_table(int dims, int iso, int* off, int* kidx, int* nalt, int* altlen,
 }
 
 /* ------------------------------------------------------------------ create */
+/* ------------------------------------------------------- boundary set-up */
+/* faces: nodes on face f and the index of (x, y, z) among them, the
+ * reference's xm_vals(:, j, k) / ym_vals(:, i, k) / zm_vals(:, i, j) order */
+static long face_nodes(const orc_t* o, int f) {
+  switch (f / 2) {
+    case 0: return (long)o->NY * o->NZ;
+    case 1: return (long)o->NX * o->NZ;
+    default: return (long)o->NX * o->NY;
+  }
+}
+static long face_index(const orc_t* o, int f, int x, int y, int z) {
+  switch (f / 2) {
+    case 0: return (y - 1) + (long)o->NY * (z - 1);
+    case 1: return (x - 1) + (long)o->NX * (z - 1);
+    default: return (x - 1) + (long)o->NX * (y - 1);
+  }
+}
+/* vals(m, d) of face node i */
+#define BCV(o, f, i, d, m) (o)->bcv[f][((i) * (o)->D + (d)) * (o)->C + (m)]
+
+/* bc%flags type of a flow%bc_flags code, as FlowParseBC pairs them
+ * (lbm_flow.F90:1031-1309) */
+static int bc_type_of(int flag) {
+  switch (flag) {
+    case TXG_BC_REFLECTING: return 2;
+    case TXG_BC_DENSITY:
+    case TXG_BC_PRESSURE:
+    case TXG_BC_PRESSURE_INLET:
+    case TXG_BC_PRESSURE_OUTLET: return 3;
+    case TXG_BC_MOMENTUM:
+    case TXG_BC_FLUX:
+    case TXG_BC_FLUX_INLET:
+    case TXG_BC_FLUX_OUTLET: return 4;
+    case TXG_BC_VELOCITY: return 5;
+    default: return -1;
+  }
+}
+
+/* FlowUpdateDensityFromPressure (lbm_flow.F90:2235-2263); gf(1) of
+ * component 2 is g_21 */
+static void density_from_pressure(const orc_t* o, double pressure, double rho1frac, double* rho) {
+  const double eps = 1.e-10;
+  if (o->C == 1) {
+    rho[0] = pressure * 3.0;
+  } else if (rho1frac < eps) {
+    rho[0] = 0.0;
+    rho[1] = pressure * 3.0;
+  } else if (rho1frac > 1 - eps) {
+    rho[0] = pressure * 3.0;
+    rho[1] = 0.0;
+  } else {
+    const double alpha = 1.0 / (1.0 / rho1frac - 1.0);
+    const double g21 = o->d.g[1][0];
+    rho[0] = (-(1.0 + alpha) / 3.0 +
+              sqrt((1.0 + alpha) / 3.0 * (1.0 + alpha) / 3.0 + 4. * o->c0 * g21 * alpha * pressure)) /
+             (2 * o->c0 * g21);
+    rho[1] = rho[0] / alpha;
+  }
+}
+
+/* FlowParseBC's checks (lbm_flow.F90:1031-1309) and FlowSetUpBCsD3/D2
+ * (:1311-1956): the face values vals(1:C, 1:D) of every node of a face,
+ * from the option values.  The value arrays start zeroed (PETSc vectors). */
+static const char* setup_bcs(orc_t* o) {
+  const txg_desc* d = &o->d;
+  const int C = o->C, D = o->D;
+  for (int f = 0; f < 6; f++) {
+    const int dim = f / 2, plus = f & 1, flag = d->bc_flags[f];
+    o->bckind[f] = 0;
+    if (dim >= D || flag == TXG_BC_NULL) continue;
+    const int t = bc_type_of(flag);
+    if (t < 0) return "unknown boundary condition code";
+    if (d->periodic[dim]) return "boundary condition on a periodic face";
+    if ((flag == TXG_BC_PRESSURE || flag == TXG_BC_FLUX) && C != 1)
+      return "pressure and flux BCs take one component";
+    if ((flag == TXG_BC_PRESSURE_INLET || flag == TXG_BC_FLUX_INLET) && C > 2)
+      return "pressure and flux inlets are implemented for 1 or 2 components";
+    if (flag == TXG_BC_PRESSURE_OUTLET && C != 1)
+      return "a pressure outlet of several components is not implemented";
+    if (flag == TXG_BC_REFLECTING && D == 2 && f == 0)
+      return "a reflecting xm face in 2-D is not implemented";
+    o->bckind[f] = t;
+    if (t == 2) continue;
+    const long nf = face_nodes(o, f);
+    o->bcn[f] = nf;
+    o->bcv[f] = (double*)calloc((size_t)(nf * D * C), sizeof(double));
+    if (!o->bcv[f]) return "out of memory";
+    double v[3 * MAXC];
+    memset(v, 0, sizeof(v));
+    const double* bd = d->bc_data[f];
+    switch (flag) {
+      case TXG_BC_DENSITY:
+        for (int m = 0; m < C; m++) v[m] = d->bc_density[f][m];
+        break;
+      case TXG_BC_PRESSURE_INLET:
+      case TXG_BC_PRESSURE: {
+        double rho[MAXC] = {0.0, 0.0, 0.0, 0.0};
+        density_from_pressure(o, bd[0], bd[1], rho);
+        for (int m = 0; m < C; m++) v[m] = rho[m];
+        break;
+      }
+      case TXG_BC_PRESSURE_OUTLET:
+        v[0] = bd[0];
+        break;
+      case TXG_BC_MOMENTUM:
+        for (int m = 0; m < C; m++)
+          for (int dd = 0; dd < D; dd++) v[dd * C + m] = bd[m * D + dd];
+        break;
+      case TXG_BC_FLUX_INLET:
+      case TXG_BC_FLUX:
+        if (C == 1) {
+          v[dim * C] = plus ? -bd[0] : bd[0];
+        } else {
+          v[dim * C] = bd[0] * bd[1];
+          v[dim * C + 1] = bd[0] * (1.0 - bd[1]);
+        }
+        break;
+      case TXG_BC_FLUX_OUTLET:
+        v[dim * C] = plus ? -bd[0] : bd[0];
+        break;
+      case TXG_BC_VELOCITY:
+        for (int dd = 0; dd < D; dd++) v[dd * C] = bd[dd];
+        break;
+    }
+    for (long i = 0; i < nf; i++) memcpy(o->bcv[f] + i * D * C, v, sizeof(double) * (size_t)(D * C));
+  }
+  return NULL;
+}
+
+/* BCSetValues with an initialize_bcs callback (lbm_bc.F90:215-228) */
+int orc_set_bc_values(orc_t* o, int face, const double* vals) {
+  if (face < 0 || face >= 2 * o->D || !o->bcv[face]) return TXG_ERR_ARG;
+  memcpy(o->bcv[face], vals, sizeof(double) * (size_t)(o->bcn[face] * o->D * o->C));
+  return 0;
+}
+
 orc_t* orc_create(const txg_desc* d, char* err, int errlen) {
 #define FAIL(msg)                                  \
   do {                                             \
@@ -464,15 +613,11 @@ orc_t* orc_create(const txg_desc* d, char* err, int errlen) {
     for (int m = 0; m < o->C; m++)
       if (fabs(d->gw[k][m]) > eps15) o->fs = 1;
   o->body = d->body_forces != 0;
-  for (int f = 0; f < 6; f++) {
-    int dim = f / 2;
-    if (dim >= o->D) continue;
-    if (d->bc_flags[f] == TXG_BC_DENSITY) {
-      if (d->periodic[dim]) FAIL("density BC on a periodic face");
-      o->dirichlet[f] = 1;
-    } else if (d->bc_flags[f] != TXG_BC_NULL) {
-      FAIL("unsupported boundary condition kind");
-    }
+  const char* bcerr = setup_bcs(o);
+  if (bcerr) {
+    if (err) snprintf(err, (size_t)errlen, "%s", bcerr);
+    orc_destroy(o);
+    return NULL;
   }
   return o;
 #undef FAIL
@@ -488,6 +633,7 @@ void orc_destroy(orc_t* o) {
   free(o->walls);
   free(o->flux);
   free(o->forces);
+  for (int f = 0; f < 6; f++) free(o->bcv[f]);
   free(o);
 }
 
@@ -833,13 +979,17 @@ static void collide(orc_t* o) {
 }
 
 /* ------------------------------------------------------------ streaming */
-/* BCPreStream_D2/D3 (lbm_bc.F90:613-779): Dirichlet faces copy the
- * boundary node's outgoing-from-outside populations into the ghost source
- * cells so that the stream delivers f*(x, t). */
+/* faces whose BC type is Dirichlet, Neumann or velocity (the ones
+ * BCPreStream and BCUpdateRho visit, lbm_bc.F90:436-455, :613-631) */
+static int node_bc_face(const orc_t* o, int f) { return o->bckind[f] >= 3; }
+
+/* BCPreStream_D2/D3 (lbm_bc.F90:613-779): Dirichlet, Neumann and velocity
+ * faces copy the boundary node's outgoing-from-outside populations into
+ * the ghost source cells so that the stream delivers f*(x, t). */
 static void bc_prestream(orc_t* o) {
   int lo[3] = {1, 1, 1}, hi[3] = {o->NX, o->NY, o->NZ};
   for (int f = 0; f < 2 * o->D; f++) {
-    if (!o->dirichlet[f]) continue;
+    if (!node_bc_face(o, f)) continue;
     int dim = f / 2, plus = f % 2;
     for (int n = 1; n <= o->b; n++) {
       int c = o->ci[n][dim];
@@ -940,14 +1090,16 @@ static void face_range(const orc_t* o, int f, int a0[3], int a1[3]) {
 /* BCApplyDirichletToRho_D2/D3 (lbm_bc.F90:252-434) */
 static void bc_dirichlet_to_rho(orc_t* o) {
   for (int f = 0; f < 2 * o->D; f++) {
-    if (!o->dirichlet[f]) continue;
+    if (o->bckind[f] != 3) continue;
     int a0[3], a1[3];
     face_range(o, f, a0, a1);
     for (int z = a0[2]; z <= a1[2]; z++)
       for (int y = a0[1]; y <= a1[1]; y++)
         for (int x = a0[0]; x <= a1[0]; x++)
-          if (WALL(o, x, y, z) == 0)
-            for (int m = 0; m < o->C; m++) RHO(o, x, y, z, m) = o->d.bc_density[f][m];
+          if (WALL(o, x, y, z) == 0) {
+            const long i = face_index(o, f, x, y, z);
+            for (int m = 0; m < o->C; m++) RHO(o, x, y, z, m) = BCV(o, f, i, 0, m);
+          }
   }
 }
 
@@ -970,19 +1122,20 @@ static void local_dirs(const orc_t* o, int f, int* ln, int card[3]) {
   }
 }
 
-/* BCApplyDirichletNode (lbm_bc.F90:1273-1333) */
+/* BCApplyDirichletNode (lbm_bc.F90:1273-1333): pvals(m, 1) is the density */
 static void dirichlet_node(orc_t* o, int x, int y, int z, int f) {
   int ln, card[3];
   local_dirs(o, f, &ln, card);
   int nrm = card[0];
   int cl = o->ci[ln][nrm];
+  const long i = face_index(o, f, x, y, z);
   for (int m = 0; m < o->C; m++) {
     double wsum[3] = {0., 0., 0.}, Q[3] = {0., 0., 0.}, mom[3] = {0., 0., 0.};
     for (int n = 1; n <= o->b; n++)
       if (cl * o->ci[n][nrm] == 1) wsum[nrm] = wsum[nrm] + o->w[n];
     double s = 0.0;
     for (int n = 0; n <= o->b; n++) s = s + FI(o, x, y, z, n, m);
-    Q[nrm] = (double)cl * (o->d.bc_density[f][m] - s) / wsum[nrm];
+    Q[nrm] = (double)cl * (BCV(o, f, i, 0, m) - s) / wsum[nrm];
     for (int p = 1; p < o->D; p++) {
       int cp = card[p];
       for (int n = 1; n <= o->b; n++) {
@@ -1000,23 +1153,118 @@ static void dirichlet_node(orc_t* o, int x, int y, int z, int f) {
   }
 }
 
-/* BCApplyDirichletD2/D3 (lbm_bc.F90:1090-1271) */
+/* BCApplyNeumannNode (lbm_bc.F90:1533-1593): mvals(m, p) is the momentum
+ * the node's populations carry after the correction, j + F/2 */
+static void neumann_node(orc_t* o, int x, int y, int z, int f) {
+  int ln, card[3];
+  local_dirs(o, f, &ln, card);
+  const int nrm = card[0], cl = o->ci[ln][nrm];
+  const long i = face_index(o, f, x, y, z);
+  for (int m = 0; m < o->C; m++) {
+    double wsum[3] = {0., 0., 0.}, Q[3] = {0., 0., 0.}, mom[3] = {0., 0., 0.};
+    for (int p = 0; p < o->D; p++) {
+      for (int n = 1; n <= o->b; n++) {
+        mom[p] = mom[p] + FI(o, x, y, z, n, m) * (double)o->ci[n][p];
+        if (cl * o->ci[n][nrm] == 1 && o->ci[n][p] != 0) wsum[p] = wsum[p] + o->w[n];
+      }
+      Q[p] = (BCV(o, f, i, p, m) - FRC(o, x, y, z, p, m) / 2.0 - mom[p]) / wsum[p];
+    }
+    for (int n = 1; n <= o->b; n++)
+      if (cl * o->ci[n][nrm] == 1) {
+        double cq = 0.0;
+        for (int d = 0; d < o->D; d++) cq = cq + (double)o->ci[n][d] * Q[d];
+        FI(o, x, y, z, n, m) = FI(o, x, y, z, n, m) + o->w[n] * cq;
+      }
+  }
+}
+
+/* BCApplyVelocityNode (lbm_bc.F90:1793-1865): uvals(1, p) is the velocity
+ * of every component; the normal correction also fixes the density */
+static void velocity_node(orc_t* o, int x, int y, int z, int f) {
+  int ln, card[3];
+  local_dirs(o, f, &ln, card);
+  const int nrm = card[0], cl = o->ci[ln][nrm];
+  const long i = face_index(o, f, x, y, z);
+  for (int m = 0; m < o->C; m++) {
+    double wsum[3] = {0., 0., 0.}, Q[3] = {0., 0., 0.}, mom[3] = {0., 0., 0.};
+    for (int n = 1; n <= o->b; n++) {
+      mom[nrm] = mom[nrm] + FI(o, x, y, z, n, m) * (double)o->ci[n][nrm];
+      if (cl * o->ci[n][nrm] == 1) wsum[nrm] = wsum[nrm] + o->w[n];
+    }
+    double s = 0.0;
+    for (int n = 0; n <= o->b; n++) s = s + FI(o, x, y, z, n, m);
+    const double un = BCV(o, f, i, nrm, 0);
+    Q[nrm] = (s * un - mom[nrm] - FRC(o, x, y, z, nrm, m) / 2.0) / (1.0 - (double)cl * un) / wsum[nrm];
+    const double rho = s + wsum[nrm] * Q[nrm] * (double)cl;
+    for (int p = 1; p < o->D; p++) {
+      const int cp = card[p];
+      for (int n = 1; n <= o->b; n++) {
+        mom[cp] = mom[cp] + FI(o, x, y, z, n, m) * (double)o->ci[n][cp];
+        if (cl * o->ci[n][nrm] == 1 && o->ci[n][cp] != 0) wsum[cp] = wsum[cp] + o->w[n];
+      }
+      Q[cp] = (rho * BCV(o, f, i, cp, 0) - FRC(o, x, y, z, cp, m) / 2.0 - mom[cp]) / wsum[cp];
+    }
+    for (int n = 1; n <= o->b; n++)
+      if (cl * o->ci[n][nrm] == 1) {
+        double cq = 0.0;
+        for (int d = 0; d < o->D; d++) cq = cq + (double)o->ci[n][d] * Q[d];
+        FI(o, x, y, z, n, m) = FI(o, x, y, z, n, m) + o->w[n] * cq;
+      }
+  }
+}
+
+/* BCApplyReflectingD3/D2 (lbm_bc.F90:825-1088): every inward link n takes,
+ * in turn, the value of each link p that matches it along the in-face axes
+ * and is its mirror along the normal.  On the xm face of D3Q19 the
+ * reference compares c_n,x with -c_p,z instead of -c_p,x (lbm_bc.F90:848),
+ * so there only the link (1,0,-1) changes, to the last match (-1,0,-1);
+ * that comparison is kept. */
+static void reflect_node(orc_t* o, int x, int y, int z, int f) {
+  const int dim = f / 2, sgn = (f & 1) ? -1 : 1, pa = (f == 0) ? 2 : dim;
+  for (int n = 1; n <= o->b; n++) {
+    if (o->ci[n][dim] * sgn <= 0) continue;
+    for (int p = 1; p <= o->b; p++) {
+      int ok = o->ci[n][dim] == -o->ci[p][pa];
+      for (int a = 0; a < o->D; a++)
+        if (a != dim && o->ci[n][a] != o->ci[p][a]) ok = 0;
+      if (ok)
+        for (int m = 0; m < o->C; m++) FI(o, x, y, z, n, m) = FI(o, x, y, z, p, m);
+    }
+  }
+}
+
+/* BCApply (lbm_bc.F90:781-807): each BC type once, in the order the faces
+ * first name it; a type's routine visits its faces xm..zp
+ * (BCApplyDirichletD3 :1090-1271, BCApplyNeumannD3 :1351-1462,
+ * BCApplyVelocityD3 :1611-1722, BCApplyReflectingD3 :825-977), pore nodes
+ * only */
 static void bc_apply(orc_t* o) {
-  for (int f = 0; f < 2 * o->D; f++) {
-    if (!o->dirichlet[f]) continue;
-    int a0[3], a1[3];
-    face_range(o, f, a0, a1);
-    for (int z = a0[2]; z <= a1[2]; z++)
-      for (int y = a0[1]; y <= a1[1]; y++)
-        for (int x = a0[0]; x <= a1[0]; x++)
-          if (WALL(o, x, y, z) == 0) dirichlet_node(o, x, y, z, f);
+  int done[6] = {0, 0, 0, 0, 0, 0};
+  for (int s = 0; s < 2 * o->D; s++) {
+    const int t = o->bckind[s];
+    if (!t || done[t]) continue;
+    done[t] = 1;
+    for (int f = 0; f < 2 * o->D; f++) {
+      if (o->bckind[f] != t) continue;
+      int a0[3], a1[3];
+      face_range(o, f, a0, a1);
+      for (int z = a0[2]; z <= a1[2]; z++)
+        for (int y = a0[1]; y <= a1[1]; y++)
+          for (int x = a0[0]; x <= a1[0]; x++) {
+            if (WALL(o, x, y, z) != 0) continue;
+            if (t == 2) reflect_node(o, x, y, z, f);
+            else if (t == 3) dirichlet_node(o, x, y, z, f);
+            else if (t == 4) neumann_node(o, x, y, z, f);
+            else velocity_node(o, x, y, z, f);
+          }
+    }
   }
 }
 
 /* BCUpdateRho_D2/D3 (lbm_bc.F90:436-611) */
 static void bc_update_rho(orc_t* o) {
   for (int f = 0; f < 2 * o->D; f++) {
-    if (!o->dirichlet[f]) continue;
+    if (!node_bc_face(o, f)) continue;
     int a0[3], a1[3];
     face_range(o, f, a0, a1);
     for (int z = a0[2]; z <= a1[2]; z++)

@@ -27,6 +27,7 @@ void orc_destroy(orc_t* o);
 int orc_set_walls(orc_t* o, const double* walls);          /* owned, natural */
 int orc_set_state(orc_t* o, const double* rho, const double* u);
 int orc_set_fi(orc_t* o, const double* fi);
+int orc_set_bc_values(orc_t* o, int face, const double* vals); /* [(i*D+d)*C+m] */
 int orc_init(orc_t* o);                                      /* LBMInit2 flow part */
 int orc_step(orc_t* o, int nsteps);                          /* LBMRun2 flow body */
 int orc_get_fi(const orc_t* o, double* fi);

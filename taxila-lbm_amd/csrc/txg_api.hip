@@ -105,10 +105,10 @@ inline int cdiv(long a, int b) { return (int)((a + b - 1) / b); }
 // Per pore node: bit n (1..Q-1) when the pull source x - c_n is a wall
 // (bounce-back link), kFSNear when some neighbour x + c_n is a mineral
 // (wettability link), kFFClean when no wall lies within the gradient
-// stencil's reach, bits kFaceShift+f on Dirichlet faces; solids get
+// stencil's reach, bits kFaceShift+f on faces with a BC; solids get
 // kSelfSolid only.
 struct MaskGeom {
-  int nx, ny, nz, per[3], nminerals, reach, dirichlet, N;
+  int nx, ny, nz, per[3], nminerals, reach, faces, N;
 };
 
 template <int D>
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) k_build_mask(const double* __restrict__ w
   if (clean) mk |= kFFClean;
   const int pos[3] = {x, y, z}, hi[3] = {g.nx - 1, g.ny - 1, g.nz - 1};
   for (int f = 0; f < 2 * D; f++) {
-    if (!(g.dirichlet & (1 << f))) continue;
+    if (!(g.faces & (1 << f))) continue;
     const int dim = f / 2;
     if (pos[dim] == ((f & 1) ? hi[dim] : 0)) mk |= 1u << (kFaceShift + f);
   }
@@ -199,6 +199,10 @@ struct txg_handle {
   void* u_ic = nullptr;  // R[N*D*C]
   void* shift = nullptr; // R[C*Q] storage shifts for k_load_fi
   void* f0[2] = {nullptr, nullptr};  // wide rest populations (FP32: double, FP16: float storage)
+  void* bcv = nullptr;               // R: per-node face values of the BC faces (Params::bcv)
+  std::vector<double> hbcv[6];       // the same values in FP64, per face
+  int bcoff[6] = {0, 0, 0, 0, 0, 0};
+  int has_bc = 0;                    // some face has a boundary condition
   // FP64 export arrays
   double* ex = nullptr;
   size_t ex_len = 0;
@@ -294,6 +298,149 @@ AnyBufs bufs(txg_handle* h, int in) {
   return a;
 }
 
+// the first tile shape with a kernel in a KernelSet array (0 if none)
+template <class P>
+int first_tile(const P (&fns)[kNumTiles]) {
+  for (int t = 0; t < kNumTiles; t++)
+    if (fns[t]) return t;
+  return 0;
+}
+
+// Face boundary conditions of a descriptor: the BC type of each face, the
+// order BCApply visits them and the per-node face values vals(1:C, 1:D)
+// (FlowParseBC lbm_flow.F90:1031-1309 and FlowSetUpBCsD3/D2 :1311-1956;
+// the value vectors start zeroed like PETSc's).
+struct BcSetup {
+  int kind[6] = {0, 0, 0, 0, 0, 0};
+  int dirichlet = 0;
+  std::vector<double> vals[6];
+};
+
+// bc%flags type of a flow%bc_flags code
+int bc_type_of(int flag) {
+  switch (flag) {
+    case TXG_BC_REFLECTING: return kBcReflect;
+    case TXG_BC_DENSITY:
+    case TXG_BC_PRESSURE:
+    case TXG_BC_PRESSURE_INLET:
+    case TXG_BC_PRESSURE_OUTLET: return kBcDirichlet;
+    case TXG_BC_MOMENTUM:
+    case TXG_BC_FLUX:
+    case TXG_BC_FLUX_INLET:
+    case TXG_BC_FLUX_OUTLET: return kBcNeumann;
+    case TXG_BC_VELOCITY: return kBcVelocity;
+    default: return -1;
+  }
+}
+
+// FlowUpdateDensityFromPressure (lbm_flow.F90:2235-2263); gf(1) of
+// component 2 is g_21, c_0 = 6 (lbm_discretization_d3q19.F90:170)
+void density_from_pressure(const txg_desc& d, double pressure, double rho1frac, double* rho) {
+  const double eps = 1.e-10, c0 = 6.0;
+  if (d.ncomponents == 1) {
+    rho[0] = pressure * 3.0;
+  } else if (rho1frac < eps) {
+    rho[0] = 0.0;
+    rho[1] = pressure * 3.0;
+  } else if (rho1frac > 1 - eps) {
+    rho[0] = pressure * 3.0;
+    rho[1] = 0.0;
+  } else {
+    const double alpha = 1.0 / (1.0 / rho1frac - 1.0);
+    const double g21 = d.g[1][0];
+    rho[0] = (-(1.0 + alpha) / 3.0 + std::sqrt((1.0 + alpha) / 3.0 * (1.0 + alpha) / 3.0 + 4. * c0 * g21 * alpha * pressure)) /
+             (2 * c0 * g21);
+    rho[1] = rho[0] / alpha;
+  }
+}
+
+int bc_setup(const txg_desc& d, BcSetup& b, const char** msg) {
+  const int C = d.ncomponents, D = d.ndims;
+  const long n3[3] = {d.nx, d.ny, D == 3 ? d.nz : 1};
+  for (int f = 0; f < 6; f++) {
+    const int dim = f / 2, plus = f & 1, flag = d.bc_flags[f];
+    if (dim >= D || flag == TXG_BC_NULL) continue;
+    const int t = bc_type_of(flag);
+    *msg = nullptr;
+    if (t < 0) *msg = "unknown boundary condition code";
+    else if (d.periodic[dim]) *msg = "boundary condition on a periodic face";
+    if (*msg) return TXG_ERR_ARG;
+    if ((flag == TXG_BC_PRESSURE || flag == TXG_BC_FLUX) && C != 1) *msg = "pressure and flux BCs take one component";
+    else if ((flag == TXG_BC_PRESSURE_INLET || flag == TXG_BC_FLUX_INLET) && C > 2)
+      *msg = "pressure and flux inlets are implemented for 1 or 2 components";
+    else if (flag == TXG_BC_PRESSURE_OUTLET && C != 1)
+      *msg = "a pressure outlet of several components (rho_1/rho_t updated every step) is not implemented";
+    else if (flag == TXG_BC_REFLECTING && D == 2 && f == 0)
+      *msg = "a reflecting xm face in 2-D is not implemented (the reference reads outside its lattice table there)";
+    if (*msg) return TXG_ERR_UNSUPPORTED;
+    b.kind[f] = t;
+    if (t == kBcDirichlet) b.dirichlet |= 1 << f;
+    if (t == kBcReflect) continue;
+    double v[3 * TXG_MAX_COMPONENTS] = {};
+    const double* bd = d.bc_data[f];
+    switch (flag) {
+      case TXG_BC_DENSITY:
+        for (int m = 0; m < C; m++) v[m] = d.bc_density[f][m];
+        break;
+      case TXG_BC_PRESSURE_INLET:
+      case TXG_BC_PRESSURE: {
+        double rho[TXG_MAX_COMPONENTS] = {};
+        density_from_pressure(d, bd[0], bd[1], rho);
+        for (int m = 0; m < C; m++) v[m] = rho[m];
+        break;
+      }
+      case TXG_BC_PRESSURE_OUTLET:
+        v[0] = bd[0];
+        break;
+      case TXG_BC_MOMENTUM:
+        for (int m = 0; m < C; m++)
+          for (int k = 0; k < D; k++) v[k * C + m] = bd[m * D + k];
+        break;
+      case TXG_BC_FLUX_INLET:
+      case TXG_BC_FLUX:
+        if (C == 1) {
+          v[dim * C] = plus ? -bd[0] : bd[0];
+        } else {
+          v[dim * C] = bd[0] * bd[1];
+          v[dim * C + 1] = bd[0] * (1.0 - bd[1]);
+        }
+        break;
+      case TXG_BC_FLUX_OUTLET:
+        v[dim * C] = plus ? -bd[0] : bd[0];
+        break;
+      case TXG_BC_VELOCITY:
+        for (int k = 0; k < D; k++) v[k * C] = bd[k];
+        break;
+    }
+    const long nf = dim == 0 ? n3[1] * n3[2] : (dim == 1 ? n3[0] * n3[2] : n3[0] * n3[1]);
+    b.vals[f].resize((size_t)nf * D * C);
+    for (long i = 0; i < nf; i++) std::copy(v, v + D * C, b.vals[f].begin() + i * D * C);
+  }
+  return TXG_OK;
+}
+
+// BC fields of Params: types, BCApply order (by type in the order the
+// faces first name it, then xm..zp within a type), BCUpdateRho faces and
+// the value offsets
+template <class R>
+void bc_params(const BcSetup& b, const int (&off)[6], Params<R>& p) {
+  p.bckinds = p.bcseq = p.nbcseq = p.bcupd = 0;
+  for (int f = 0; f < 6; f++) {
+    p.bckinds |= b.kind[f] << (3 * f);
+    p.bcoff[f] = off[f];
+    if (b.kind[f] == kBcDirichlet || b.kind[f] == kBcNeumann || b.kind[f] == kBcVelocity) p.bcupd |= 1 << f;
+  }
+  bool done[8] = {};
+  for (int s = 0; s < 6; s++) {
+    const int t = b.kind[s];
+    if (!t || done[t]) continue;
+    done[t] = true;
+    for (int f = 0; f < 6; f++)
+      if (b.kind[f] == t) p.bcseq |= f << (3 * p.nbcseq++);
+  }
+  p.dirichlet = b.dirichlet;
+}
+
 template <class R>
 void fill_params(const txg_desc& d, int N, int Np, int flags, int dirichlet, Params<R>& p) {
   memset(&p, 0, sizeof(p));
@@ -308,6 +455,7 @@ void fill_params(const txg_desc& d, int N, int Np, int flags, int dirichlet, Par
   p.flags = flags;
   p.nminerals = d.nminerals;
   p.dirichlet = dirichlet;
+  p.bcv = nullptr;
   const int C = d.ncomponents;
   for (int m = 0; m < C; m++) {
     const double cs2 = 1.0 / 3.;                 // lbm_relaxation.F90:79
@@ -332,7 +480,6 @@ void fill_params(const txg_desc& d, int N, int Np, int flags, int dirichlet, Par
     for (int k = 0; k < 7; k++) p.rate[m][k] = (R)rates[k];
     for (int n = 0; n < C; n++) p.g[m][n] = (R)d.g[m][n];
     for (int k = 0; k < d.nminerals; k++) p.gw[k][m] = (R)d.gw[k][m];
-    for (int f = 0; f < 6; f++) p.bcrho[f][m] = (R)d.bc_density[f][m];
     p.shift0[m] = (R)0;
   }
   for (int i = 0; i < 3; i++) p.gvt[i] = (R)d.gvt[i];
@@ -465,6 +612,23 @@ int txg_desc_init(txg_desc* d) {
   return TXG_OK;
 }
 
+// copy face f's values (FP64 on the host) to the device in the compute type
+int upload_bc_values(txg_handle* h, int f) {
+  const std::vector<double>& v = h->hbcv[f];
+  int rc;
+  if (h->prec == TXG_FP64) {
+    rc = h->hip(hipMemcpyAsync((double*)h->bcv + h->bcoff[f], v.data(), v.size() * 8, hipMemcpyHostToDevice, h->stream),
+                "boundary values");
+    if (!rc) rc = h->hip(hipStreamSynchronize(h->stream), "boundary values");
+  } else {
+    std::vector<float> w(v.begin(), v.end());
+    rc = h->hip(hipMemcpyAsync((float*)h->bcv + h->bcoff[f], w.data(), w.size() * 4, hipMemcpyHostToDevice, h->stream),
+                "boundary values");
+    if (!rc) rc = h->hip(hipStreamSynchronize(h->stream), "boundary values");
+  }
+  return rc;
+}
+
 int txg_create(const txg_desc* d, txg_handle** out) {
   g_create_error.clear();
   auto bad = [](int code, const char* msg) {
@@ -491,17 +655,13 @@ int txg_create(const txg_desc* d, txg_handle** out) {
     if (!(d->mm[m] > 0)) return bad(TXG_ERR_ARG, "mm must be positive");
     if (d->relaxation_mode == TXG_SRT && !(d->tau[m] > 0.5)) return bad(TXG_ERR_ARG, "tau must exceed 1/2");
   }
-  int dirichlet = 0;
-  for (int f = 0; f < 6; f++) {
-    const int dim = f / 2;
-    if (dim >= d->ndims) continue;
-    if (d->bc_flags[f] == TXG_BC_DENSITY) {
-      if (d->periodic[dim]) return bad(TXG_ERR_ARG, "density BC on a periodic face");
-      dirichlet |= 1 << f;
-    } else if (d->bc_flags[f] != TXG_BC_NULL) {
-      return bad(TXG_ERR_UNSUPPORTED, "only periodic, wall and density faces are implemented");
-    }
+  BcSetup bcs;
+  {
+    const char* msg = nullptr;
+    const int code = bc_setup(*d, bcs, &msg);
+    if (code) return bad(code, msg);
   }
+  const int dirichlet = bcs.dirichlet;
 
   txg_handle* h = new txg_handle();
   h->d = *d;
@@ -534,6 +694,24 @@ int txg_create(const txg_desc* d, txg_handle** out) {
   fill_params<float>(*d, h->N, h->Np, flags, dirichlet, h->p32);
   if (h->prec == TXG_FP32_FP16)
     for (int m = 0; m < h->C; m++) h->p32.shift0[m] = (float)d->rho0[m];
+  // domains with BC faces step with the kernel variants that carry the face code
+  for (int f = 0; f < 6; f++) h->has_bc |= bcs.kind[f] != 0;
+  if (h->has_bc) {
+    for (int t = 0; t < kNumTiles; t++) {
+      h->ks.fused[t] = h->ks.fused_bc[t];
+      h->ks.fused_batch[t] = h->ks.fused_batch_bc[t];
+      h->ks.fused_batch_occupancy[t] = h->ks.fused_batch_bc_occupancy[t];
+    }
+    h->tile = first_tile(h->ks.fused);
+  }
+  size_t bc_len = 0;
+  for (int f = 0; f < 6; f++) {
+    h->hbcv[f] = std::move(bcs.vals[f]);
+    h->bcoff[f] = (int)bc_len;
+    bc_len += h->hbcv[f].size();
+  }
+  for (Params<double>* pp : {&h->p64}) bc_params(bcs, h->bcoff, *pp);
+  for (Params<float>* pp : {&h->p32}) bc_params(bcs, h->bcoff, *pp);
 
   // non-blocking: the handle's stream never waits for the legacy default
   // stream (nor it for the handle), so handles and batches overlap
@@ -551,6 +729,7 @@ int txg_create(const txg_desc* d, txg_handle** out) {
                   h->alloc((void**)&h->sums, (size_t)(kSumBlocks + 1) * h->ks.nsums * sizeof(double));
   const size_t rest_bytes = (size_t)h->C * h->Np * (h->prec == TXG_FP32 ? 8 : 4);
   if (alloc_ok && h->prec != TXG_FP64) alloc_ok = h->alloc(&h->f0[0], rest_bytes) && h->alloc(&h->f0[1], rest_bytes);
+  if (alloc_ok && bc_len) alloc_ok = h->alloc(&h->bcv, bc_len * R);
   if (!alloc_ok || hipStreamSynchronize(h->stream) != hipSuccess) {
     txg_destroy(h);
     return bad(TXG_ERR_NOMEM, "device allocation failed");
@@ -582,12 +761,27 @@ int txg_create(const txg_desc* d, txg_handle** out) {
       (void)hipStreamSynchronize(h->stream);
     }
   }
+  h->p64.bcv = (const double*)h->bcv;
+  h->p32.bcv = (const float*)h->bcv;
+  for (int f = 0; f < 6; f++)
+    if (!h->hbcv[f].empty() && upload_bc_values(h, f)) {
+      txg_destroy(h);
+      return bad(TXG_ERR_HIP, "boundary value upload failed");
+    }
   if (hipStreamSynchronize(h->stream) != hipSuccess) {
     txg_destroy(h);
     return bad(TXG_ERR_HIP, "initialisation failed");
   }
   *out = h;
   return TXG_OK;
+}
+
+int txg_set_bc_values(txg_handle* h, int32_t face, const double* vals) {
+  if (!h || !vals) return TXG_ERR_ARG;
+  if (face < 0 || face >= 2 * h->D) return h->fail(TXG_ERR_ARG, "face %d out of range", face);
+  if (h->hbcv[face].empty()) return h->fail(TXG_ERR_ARG, "face %d has no boundary values", face);
+  std::copy(vals, vals + h->hbcv[face].size(), h->hbcv[face].begin());
+  return upload_bc_values(h, face);
 }
 
 int txg_set_walls(txg_handle* h, const double* walls) {
@@ -608,7 +802,9 @@ int txg_set_walls(txg_handle* h, const double* walls) {
   for (int k = 0; k < 3; k++) g.per[k] = k < D ? d.periodic[k] : 1;
   g.nminerals = d.nminerals;
   g.reach = reach;
-  g.dirichlet = h->p64.dirichlet;
+  g.faces = 0;
+  for (int f = 0; f < 2 * D; f++)
+    if ((h->p64.bckinds >> (3 * f)) & 7) g.faces |= 1 << f;
   g.N = N;
   double* dw = nullptr;
   int rc = h->hip(hipMallocAsync((void**)&dw, (size_t)N * 8, h->stream), "walls staging");
@@ -884,8 +1080,9 @@ int txg_tune(txg_handle* h, int32_t kernel, int32_t tile_x, int32_t tile_y, int3
       if (kTileX[i] == tile_x && kTileY[i] == tile_y) tile = i;
     if (tile < 0) return h->fail(TXG_ERR_UNSUPPORTED, "tile %dx%d is not instantiated", tile_x, tile_y);
   }
-  if (kernel == 2 && !h->ks.fused[tile])
-    return h->fail(TXG_ERR_UNSUPPORTED, "no one-pass kernel for this lattice");
+  if ((kernel == 2 || tile_x != 0 || tile_y != 0) && !h->ks.fused[tile])
+    return h->fail(TXG_ERR_UNSUPPORTED, "no one-pass kernel of this tile shape for this handle "
+                                        "(domains with boundary-condition faces run 64x4)");
   if (kernel == 2 && (uint64_t)h->Q * h->Np * h->ssize >= (1ull << 32))
     return h->fail(TXG_ERR_UNSUPPORTED, "one-pass kernel needs < 4 GiB of populations per component");
   int rc = h->hip(hipStreamSynchronize(h->stream), "tune sync");
@@ -1097,6 +1294,7 @@ int txg_destroy(txg_handle* h) {
     h->release(h->f[1]);
     h->release(h->f0[0]);
     h->release(h->f0[1]);
+    h->release(h->bcv);
     h->release(h->rho);
     h->release(h->mask);
     h->release(h->tag);
@@ -1129,6 +1327,7 @@ struct txg_batch {
   std::vector<txg_handle*> mem;
   KernelSet ks{};
   int C = 0, iso = -1, prec = -1;
+  int bc = 0;  // ks holds the kernel variants for boundary-condition faces
   // device member table of the current launch set
   void* dmem = nullptr;
   size_t dmem_cap = 0;
@@ -1183,6 +1382,7 @@ void fill_member(const txg_batch* b, txg_handle* h, const Params<R>& p, BatchMem
 
 // write the device table for `set` (members in pulled state); flip restarts
 int write_table(txg_batch* b, const std::vector<txg_handle*>& set) {
+  if (!b->ks.fused_batch[b->tile]) b->tile = first_tile(b->ks.fused_batch);
   const size_t mb = b->ks.member_bytes;
   std::vector<unsigned char> host(mb * std::max<size_t>(1, set.size()));
   std::vector<int> prefix(set.size() + 1, 0);
@@ -1347,7 +1547,7 @@ int txg_batch_add(txg_batch* b, txg_handle* h) {
   if (!b || !h) return TXG_ERR_ARG;
   if (h->batch) return b->fail(TXG_ERR_STATE, "handle already belongs to a batch");
   if (!h->initialized) return b->fail(TXG_ERR_STATE, "txg_init has not been called on the handle");
-  if (h->D != 3 || !h->ks.fused_batch[0])
+  if (h->D != 3 || !h->ks.fused_batch[first_tile(h->ks.fused_batch)])
     return b->fail(TXG_ERR_UNSUPPORTED, "batched steps need a D3Q19 handle");
   if ((uint64_t)h->Q * h->Np * h->ssize >= (1ull << 32))
     return b->fail(TXG_ERR_UNSUPPORTED, "batched steps need < 4 GiB of populations per component");
@@ -1357,8 +1557,14 @@ int txg_batch_add(txg_batch* b, txg_handle* h) {
     b->C = h->C;
     b->iso = iso;
     b->prec = h->prec;
+    b->bc = h->has_bc;
   } else if (h->C != b->C || iso != b->iso || h->prec != b->prec) {
     return b->fail(TXG_ERR_UNSUPPORTED, "batch members must share components, isotropy and precision");
+  } else if (h->has_bc && !b->bc) {
+    // a member with BC faces: the batch launches the variant with the face
+    // code from now on (it steps members without BC faces identically)
+    b->ks = h->ks;
+    b->bc = 1;
   }
   b->mem.push_back(h);
   h->batch = b;

@@ -236,7 +236,7 @@ struct FusedShape {
 // boundary is then where both segments start (or both end), so the halo
 // planes one segment reads beyond its range are read by the other at about
 // the same time, and the second read hits L2 instead of HBM.
-template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
+template <int C, int NFF, bool FAST, class S, class R, int TX, int TY, bool BC>
 __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>& b, int x0, int y0, int z0, int z1,
                                             int dir, R* srho) {
   using L = Lat<3>;
@@ -326,6 +326,7 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
     return b.mask[(rx[k][1] + ry[k][1] + zr[1]) / SB];
   };
 
+  const FaceRho<3, C, R> face_rho(p);
   // pull the column at rows (xr, yr, zr) with link mask mk; density
   // (Dirichlet-overridden) to rho
   auto pull_col = [&](const uint32_t (&xr)[3], const uint32_t (&yr)[3], const uint32_t (&zr)[3], uint32_t mk,
@@ -348,13 +349,8 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
       for (int n = 0; n < Q; n++) s = s + f[m][n];
       rho[m] = s;
     }
-    if (mk & (0x3fu << kFaceShift)) {
-#pragma unroll
-      for (int fc = 0; fc < 6; fc++)
-        if (mk & (1u << (kFaceShift + fc)))
-#pragma unroll
-          for (int m = 0; m < C; m++) rho[m] = p.bcrho[fc][m];
-    }
+    if constexpr (BC)
+      if (mk & (0x3fu << kFaceShift)) face_rho(mk, (int)((xr[1] + yr[1] + zr[1]) / SB), rho);
   };
 
   auto slot_of = [&](int zz) { return (zz - (z0 - RR) + NPL * 4096) % NPL; };
@@ -402,7 +398,6 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
     set_z<3, 1, R>(p, z, nb);
     const int node = nb.xw[1] + nb.yw[1] + nb.zw[1];
     if (mk & kSelfSolid) return;
-    if (mk & (0x3fu << kFaceShift)) apply_dirichlet<3, C, R>(p, mk, f);
     LdsRho<R, C, RR, EXW, EYW> acc;
     acc.s = srho;
     acc.col = ocol;
@@ -412,7 +407,7 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
 #pragma unroll
     for (int m = 0; m < C; m++) rs[m] = acc.template get<0, 0, 0>(m);
     R rl[C], j[C][3], F[C][3], up[3];
-    moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
+    moments_forces<3, C, NFF, FAST, R, BC>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, true, node);
     R res[MAXC];
     pad_res<C, R>(d0, res);
     collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, b, rout, (uint32_t)node * SB, plane_b, res), f, rl, F, up);
@@ -443,20 +438,13 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
         for (int n = 0; n < Q; n++) s2 = s2 + f[m][n];
         rs[m] = s2;
       }
-      if (mk & (0x3fu << kFaceShift)) {
-#pragma unroll
-        for (int fc = 0; fc < 6; fc++)
-          if (mk & (1u << (kFaceShift + fc)))
-#pragma unroll
-            for (int m = 0; m < C; m++) rs[m] = p.bcrho[fc][m];
-        apply_dirichlet<3, C, R>(p, mk, f);
-      }
+      if constexpr (BC) dirichlet_rho<3, C, R>(p, mk, node, rs);
       LdsRho<R, C, 0, EXW, EYW> acc;  // unused without a stencil
       acc.s = srho;
       acc.col = 0;
       acc.slot[0] = 0;
       R rl[C], j[C][3], F[C][3], up[3];
-      moments_forces<3, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
+      moments_forces<3, C, NFF, FAST, R, BC>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, true, node);
       R res[MAXC];
       pad_res<C, R>(d0, res);
       collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, b, rout, (uint32_t)node * SB, plane_b, res), f, rl, F, up);
@@ -548,7 +536,9 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
   }
 }
 
-template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
+// BC: the variant for domains with boundary-condition faces (the face code
+// costs registers, so domains without BC faces run the variant without it)
+template <int C, int NFF, bool FAST, class S, class R, int TX, int TY, bool BC>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(FusedShape<NFF, TX, TY>::template waves<R>())))
 k_fused3(Params<R> p, Bufs<S, R> b, FusedGeom g) {
   __shared__ R srho[FusedShape<NFF, TX, TY>::template lds_len<C>()];
@@ -557,8 +547,8 @@ k_fused3(Params<R> p, Bufs<S, R> b, FusedGeom g) {
   const int lb = xcd_remap(blockIdx.x, g.nblocks);
   const int seg = lb % g.nseg, tile = lb / g.nseg;
   const int z0 = seg * g.zseg;
-  fused3_item<C, NFF, FAST, S, R, TX, TY>(p, b, (tile % g.ntx) * TX, (tile / g.ntx) * TY, z0,
-                                          min(p.nz, z0 + g.zseg), (seg & 1) ? 1 : -1, srho);
+  fused3_item<C, NFF, FAST, S, R, TX, TY, BC>(p, b, (tile % g.ntx) * TX, (tile / g.ntx) * TY, z0,
+                                              min(p.nz, z0 + g.zseg), (seg & 1) ? 1 : -1, srho);
 }
 
 // one member domain of a batched one-pass launch: its parameters, its two
@@ -579,7 +569,7 @@ struct BatchMember {
 // the x-th eighth of the list with stride (workgroups on that XCD), so an
 // XCD's L2 serves contiguous tiles of the same member.  flip toggles every
 // member's orientation (every step flips all of them).
-template <int C, int NFF, bool FAST, class S, class R, int TX, int TY>
+template <int C, int NFF, bool FAST, class S, class R, int TX, int TY, bool BC>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(FusedShape<NFF, TX, TY>::template waves<R>())))
 k_fused3_batch(const BatchMember<S, R>* __restrict__ mem, const int* __restrict__ prefix, int nmem, int flip) {
   __shared__ R srho[FusedShape<NFF, TX, TY>::template lds_len<C>()];
@@ -599,9 +589,9 @@ k_fused3_batch(const BatchMember<S, R>* __restrict__ mem, const int* __restrict_
     const int lb = it - prefix[a];
     const int seg = lb % m.g.nseg, tile = lb / m.g.nseg;
     const int z0 = seg * m.g.zseg;
-    fused3_item<C, NFF, FAST, S, R, TX, TY>(m.p, m.b[m.parity ^ flip], (tile % m.g.ntx) * TX,
-                                            (tile / m.g.ntx) * TY, z0, min(m.p.nz, z0 + m.g.zseg),
-                                            (seg & 1) ? 1 : -1, srho);
+    fused3_item<C, NFF, FAST, S, R, TX, TY, BC>(m.p, m.b[m.parity ^ flip], (tile % m.g.ntx) * TX,
+                                                (tile / m.g.ntx) * TY, z0, min(m.p.nz, z0 + m.g.zseg),
+                                                (seg & 1) ? 1 : -1, srho);
   }
 }
 

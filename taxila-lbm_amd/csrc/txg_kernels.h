@@ -41,11 +41,13 @@ constexpr int MAXMIN = 16;
 constexpr uint32_t kSelfSolid = 1u << 31;   // node is not pore space
 constexpr uint32_t kFSNear = 1u << 19;      // a neighbour x + c_n is a mineral wall
 constexpr uint32_t kFFClean = 1u << 20;     // no wall inside the gradient stencil reach
-constexpr int kFaceShift = 24;               // bits 24..29: node on a Dirichlet face XM..ZP
+constexpr int kFaceShift = 24;               // bits 24..29: node on face XM..ZP, which has a BC
 // bits 1..Q-1: the pull source x - c_n is a wall -> bounce-back
 
 // flags
 constexpr int kFlagFF = 1, kFlagFS = 2, kFlagBody = 4, kFlagMRT = 8;
+// face BC types (the reference's bc%flags values, include/lbm_definitions.h:32-35)
+constexpr int kBcReflect = 2, kBcDirichlet = 3, kBcNeumann = 4, kBcVelocity = 5;
 
 template <class R>
 struct Params {
@@ -55,7 +57,15 @@ struct Params {
   int periodic[3];
   int flags;
   int nminerals;
-  int dirichlet;  // bit f set: face f has a density BC
+  int dirichlet;  // bit f set: face f has a Dirichlet (density) BC
+  // face boundary conditions: per-node face values, face f at bcv + bcoff[f]
+  // as vals[(i*D + d)*C + m] for the face's node i; bckinds holds the BC
+  // type of each face in 3 bits (kBcReflect..kBcVelocity, 0 for none);
+  // bcseq the faces in BCApply order, 3 bits each, nbcseq of them; bcupd
+  // the faces whose nodes take rho = sum f after the BCs (BCUpdateRho)
+  const R* bcv;
+  int bcoff[6];
+  int bckinds, bcseq, nbcseq, bcupd;
   R tau[MAXC];
   R mmot[MAXC];        // mm * s_c (lbm_flow.F90:518-520)
   R mm[MAXC];
@@ -73,7 +83,6 @@ struct Params {
   R c0;
   R ffeps;             // 1.e-12 as a default-real literal
   R kax, kdg;          // fluid-solid 1./6. (1./3.) and 1./12. as default reals
-  R bcrho[6][MAXC];
   R shift0[MAXC];      // FP16 storage: stored = f - w_n * shift0[m]
   R shiftw[MAXC][3];   // w * shift0[m] for the three weight classes of the
                        // lattice (rest, axis, diagonal), rounded on the host
@@ -321,13 +330,70 @@ __host__ __device__ constexpr int cdir(int n, int d) {
   return d == 0 ? Lat<D>::cx[n] : (d == 1 ? Lat<D>::cy[n] : Lat<D>::cz[n]);
 }
 
-// BCApplyDirichletNode (lbm_bc.F90:1273-1333) for face F
+// index of a node among the nodes of face F, the order of the reference's
+// xm_vals(:, j, k) / ym_vals(:, i, k) / zm_vals(:, i, j)
+template <int F, class R>
+__device__ __forceinline__ int face_index(const Params<R>& p, int node) {
+  if constexpr (F / 2 == 0) return node / p.nx;
+  else if constexpr (F / 2 == 1) return node % p.nx + p.nx * (node / p.nxy);
+  else return node % p.nxy;
+}
+
+// the face values vals(1:C, 1:D) of node `node` on face F
 template <int D, int C, int F, class R>
-__device__ __forceinline__ void dirichlet_node(const Params<R>& p, R (&f)[C][Lat<D>::Q]) {
+__device__ __forceinline__ const R* face_vals(const Params<R>& p, int node) {
+  return p.bcv + p.bcoff[F] + (size_t)face_index<F, R>(p, node) * (D * C);
+}
+
+// f_n += w_n sum_d c_n,d Q_d on the links that point inward from face F
+// (the last loop of the three node routines)
+template <int D, int C, int F, class R>
+__device__ __forceinline__ void add_incoming(R (&fm)[Lat<D>::Q], const R (&Q)[3]) {
   using L = Lat<D>;
   constexpr int ln = FaceDirs<D>::ln[F];
   constexpr int nrm = FaceDirs<D>::card[F][0];
   constexpr int cl = cdir<D>(ln, nrm);
+  sfor<1, L::Q>([&](auto NI) {
+    constexpr int n = decltype(NI)::value;
+    if constexpr (cl * cdir<D>(n, nrm) == 1) {
+      R cq = (R)0;
+      sfor<0, D>([&](auto DI) {
+        constexpr int d = decltype(DI)::value;
+        constexpr int c = cdir<D>(n, d);
+        if constexpr (c == 1) cq = cq + Q[d];
+        if constexpr (c == -1) cq = cq - Q[d];
+      });
+      fm[n] = fm[n] + (R)L::w[n] * cq;
+    }
+  });
+}
+
+// momentum of fm along axis d (sum_n f_n c_n,d over n >= 1, zero terms
+// skipped) and the weights of the inward links with c_n,d != 0
+template <int D, int F, int d, class R>
+__device__ __forceinline__ void mom_wsum(const R (&fm)[Lat<D>::Q], R& mom, R& ws) {
+  using L = Lat<D>;
+  constexpr int ln = FaceDirs<D>::ln[F];
+  constexpr int nrm = FaceDirs<D>::card[F][0];
+  constexpr int cl = cdir<D>(ln, nrm);
+  sfor<1, L::Q>([&](auto NI) {
+    constexpr int n = decltype(NI)::value;
+    constexpr int c = cdir<D>(n, d);
+    if constexpr (c == 1) mom = mom + fm[n];
+    if constexpr (c == -1) mom = mom - fm[n];
+    if constexpr (cl * cdir<D>(n, nrm) == 1 && c != 0) ws = ws + (R)L::w[n];
+  });
+}
+
+// BCApplyDirichletNode (lbm_bc.F90:1273-1333) for face F: vals(m, 1) is the
+// density of component m
+template <int D, int C, int F, class R>
+__device__ __forceinline__ void dirichlet_node(const Params<R>& p, int node, R (&f)[C][Lat<D>::Q]) {
+  using L = Lat<D>;
+  constexpr int ln = FaceDirs<D>::ln[F];
+  constexpr int nrm = FaceDirs<D>::card[F][0];
+  constexpr int cl = cdir<D>(ln, nrm);
+  const R* v = face_vals<D, C, F, R>(p, node);
 #pragma unroll
   for (int m = 0; m < C; m++) {
     R wsum[3] = {(R)0, (R)0, (R)0}, Q[3] = {(R)0, (R)0, (R)0}, mom[3] = {(R)0, (R)0, (R)0};
@@ -338,7 +404,7 @@ __device__ __forceinline__ void dirichlet_node(const Params<R>& p, R (&f)[C][Lat
     R s = (R)0;
 #pragma unroll
     for (int n = 0; n < L::Q; n++) s = s + f[m][n];
-    Q[nrm] = (R)cl * (p.bcrho[F][m] - s) / wsum[nrm];
+    Q[nrm] = (R)cl * (v[m] - s) / wsum[nrm];
     sfor<1, D>([&](auto PI) {
       constexpr int cp = FaceDirs<D>::card[F][decltype(PI)::value];
       sfor<1, L::Q>([&](auto NI) {
@@ -350,29 +416,164 @@ __device__ __forceinline__ void dirichlet_node(const Params<R>& p, R (&f)[C][Lat
       });
       Q[cp] = -mom[cp] / wsum[cp];
     });
-    sfor<1, L::Q>([&](auto NI) {
+    add_incoming<D, C, F, R>(f[m], Q);
+  }
+}
+
+// BCApplyNeumannNode (lbm_bc.F90:1533-1593) for face F: vals(m, d) is the
+// momentum j + F/2 the node carries after the correction
+template <int D, int C, int F, class R>
+__device__ __forceinline__ void neumann_node(const Params<R>& p, int node, R (&f)[C][Lat<D>::Q],
+                                             const R (&Fo)[C][3]) {
+  const R* v = face_vals<D, C, F, R>(p, node);
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R Q[3] = {(R)0, (R)0, (R)0};
+    sfor<0, D>([&](auto DI) {
+      constexpr int d = decltype(DI)::value;
+      R mom = (R)0, ws = (R)0;
+      mom_wsum<D, F, d, R>(f[m], mom, ws);
+      Q[d] = (v[d * C + m] - Fo[m][d] / (R)2 - mom) / ws;
+    });
+    add_incoming<D, C, F, R>(f[m], Q);
+  }
+}
+
+// BCApplyVelocityNode (lbm_bc.F90:1793-1865) for face F: vals(1, d) is the
+// velocity of every component; the normal correction also sets the density
+template <int D, int C, int F, class R>
+__device__ __forceinline__ void velocity_node(const Params<R>& p, int node, R (&f)[C][Lat<D>::Q],
+                                              const R (&Fo)[C][3]) {
+  using L = Lat<D>;
+  constexpr int ln = FaceDirs<D>::ln[F];
+  constexpr int nrm = FaceDirs<D>::card[F][0];
+  constexpr int cl = cdir<D>(ln, nrm);
+  const R* v = face_vals<D, C, F, R>(p, node);
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R Q[3] = {(R)0, (R)0, (R)0};
+    R mom = (R)0, ws = (R)0;
+    mom_wsum<D, F, nrm, R>(f[m], mom, ws);
+    R s = (R)0;
+#pragma unroll
+    for (int n = 0; n < L::Q; n++) s = s + f[m][n];
+    const R un = v[nrm * C];
+    Q[nrm] = (s * un - mom - Fo[m][nrm] / (R)2) / ((R)1 - (R)cl * un) / ws;
+    const R rho = s + ws * Q[nrm] * (R)cl;
+    sfor<1, D>([&](auto PI) {
+      constexpr int cp = FaceDirs<D>::card[F][decltype(PI)::value];
+      R mp = (R)0, wp = (R)0;
+      mom_wsum<D, F, cp, R>(f[m], mp, wp);
+      Q[cp] = (rho * v[cp * C] - Fo[m][cp] / (R)2 - mp) / wp;
+    });
+    add_incoming<D, C, F, R>(f[m], Q);
+  }
+}
+
+// BCApplyReflectingD3/D2 (lbm_bc.F90:825-1088) for face F: the source link
+// an inward link n ends up with after the reference's sequential loop over
+// p (the last p that matches n along the in-face axes and mirrors it along
+// the normal; on xm the reference tests c_n,x == -c_p,z, lbm_bc.F90:848),
+// or n itself when no p matches
+template <int D, int F>
+__host__ __device__ constexpr int reflect_src(int n) {
+  constexpr int dim = F / 2;
+  constexpr int pa = F == 0 ? 2 : dim;
+  int src = n;
+  for (int p = 1; p < Lat<D>::Q; p++) {
+    bool ok = cdir<D>(n, dim) == -(pa < D ? cdir<D>(p, pa) : 0);
+    for (int a = 0; a < D; a++)
+      if (a != dim && cdir<D>(n, a) != cdir<D>(p, a)) ok = false;
+    if (ok && p != n) src = p;  // p == n (f_n = f_n) keeps the value assigned so far
+  }
+  return src;
+}
+
+template <int D, int C, int F, class R>
+__device__ __forceinline__ void reflect_node(R (&f)[C][Lat<D>::Q]) {
+  constexpr int dim = F / 2, sgn = (F & 1) ? -1 : 1;
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R g[Lat<D>::Q];
+#pragma unroll
+    for (int n = 0; n < Lat<D>::Q; n++) g[n] = f[m][n];
+    sfor<1, Lat<D>::Q>([&](auto NI) {
       constexpr int n = decltype(NI)::value;
-      if constexpr (cl * cdir<D>(n, nrm) == 1) {
-        R cq = (R)0;
-        sfor<0, D>([&](auto DI) {
-          constexpr int d = decltype(DI)::value;
-          constexpr int c = cdir<D>(n, d);
-          if constexpr (c == 1) cq = cq + Q[d];
-          if constexpr (c == -1) cq = cq - Q[d];
-        });
-        f[m][n] = f[m][n] + (R)L::w[n] * cq;
+      if constexpr (cdir<D>(n, dim) * sgn > 0) {
+        constexpr int src = reflect_src<D, F>(n);
+        if constexpr (src != n) f[m][n] = g[src];
       }
     });
   }
 }
 
+// BCApply (lbm_bc.F90:781-807) on one node: the node conditions of the
+// faces the node lies on, in the order BCApply visits them (by BC type in
+// the order the faces first name the types, then xm..zp); Fo are the
+// node's forces (FlowCalcForces) the Neumann and velocity forms use
 template <int D, int C, class R>
-__device__ __forceinline__ void apply_dirichlet(const Params<R>& p, uint32_t mk, R (&f)[C][Lat<D>::Q]) {
+__device__ __forceinline__ void apply_bcs(const Params<R>& p, uint32_t mk, int node, R (&f)[C][Lat<D>::Q],
+                                          const R (&Fo)[C][3]) {
+  const uint32_t faces = (mk >> kFaceShift) & 0x3fu;
+  for (int k = 0; k < p.nbcseq; k++) {
+    const int fk = (p.bcseq >> (3 * k)) & 7;
+    if (!((faces >> fk) & 1u)) continue;
+    sfor<0, 2 * D>([&](auto FI) {
+      constexpr int F = decltype(FI)::value;
+      if (F != fk) return;
+      const int t = (p.bckinds >> (3 * F)) & 7;
+      if (t == kBcDirichlet) dirichlet_node<D, C, F, R>(p, node, f);
+      else if (t == kBcNeumann) neumann_node<D, C, F, R>(p, node, f, Fo);
+      else if (t == kBcVelocity) velocity_node<D, C, F, R>(p, node, f, Fo);
+      else if (t == kBcReflect) reflect_node<D, C, F, R>(f);
+    });
+  }
+}
+
+// the density the gradient stencil sees on a node of Dirichlet faces
+// (BCApplyDirichletToRho, lbm_bc.F90:252-434: faces xm..zp in turn)
+template <int D, int C, class R>
+__device__ __forceinline__ void dirichlet_rho(const Params<R>& p, uint32_t mk, int node, R (&rho)[C]) {
+  const uint32_t faces = (mk >> kFaceShift) & (uint32_t)p.dirichlet;
+  if (!faces) return;
   sfor<0, 2 * D>([&](auto FI) {
     constexpr int F = decltype(FI)::value;
-    if (mk & (1u << (kFaceShift + F))) dirichlet_node<D, C, F, R>(p, f);
+    if ((faces >> F) & 1u) {
+      const R* v = face_vals<D, C, F, R>(p, node);
+#pragma unroll
+      for (int m = 0; m < C; m++) rho[m] = v[m];
+    }
   });
 }
+
+// The same with the Dirichlet faces' value arrays taken out of Params once
+// per work item (the one-pass kernel's density pass over ring columns: a
+// Params access there made the compiler copy the kernel's parameter block
+// to scratch memory)
+template <int D, int C, class R>
+struct FaceRho {
+  const R* v[2 * D];  // face F's values (null unless Dirichlet)
+  int dmask, nx, nxy;
+  __device__ __forceinline__ explicit FaceRho(const Params<R>& p) : dmask(p.dirichlet), nx(p.nx), nxy(p.nxy) {
+    sfor<0, 2 * D>([&](auto FI) {
+      constexpr int F = decltype(FI)::value;
+      v[F] = ((p.dirichlet >> F) & 1) ? p.bcv + p.bcoff[F] : nullptr;
+    });
+  }
+  __device__ __forceinline__ void operator()(uint32_t mk, int node, R (&rho)[C]) const {
+    const uint32_t faces = (mk >> kFaceShift) & (uint32_t)dmask;
+    if (!faces) return;
+    sfor<0, 2 * D>([&](auto FI) {
+      constexpr int F = decltype(FI)::value;
+      if ((faces >> F) & 1u) {
+        const int i = F / 2 == 0 ? node / nx : (F / 2 == 1 ? node % nx + nx * (node / nxy) : node % nxy);
+        const R* w = v[F] + (size_t)i * (D * C);
+#pragma unroll
+        for (int m = 0; m < C; m++) rho[m] = w[m];
+      }
+    });
+  }
+};
 
 // ------------------------------------------------------------ forces
 // wall-test alternatives of a stencil offset (the rule the reference's
@@ -622,19 +823,13 @@ __global__ void __launch_bounds__(256) k_density(Params<R> p, Bufs<S, R> b, int 
     for (int n = 0; n < L::Q; n++) s = s + f[m][n];
     rho[m] = s;
   }
-  if (apply_bc && (mk & (0x3fu << kFaceShift))) {
-#pragma unroll
-    for (int fc = 0; fc < 2 * D; fc++)
-      if (mk & (1u << (kFaceShift + fc)))
-#pragma unroll
-        for (int m = 0; m < C; m++) rho[m] = p.bcrho[fc][m];
-  }
+  if (apply_bc) dirichlet_rho<D, C, R>(p, mk, node, rho);
 #pragma unroll
   for (int m = 0; m < C; m++) b.rho[(size_t)m * p.Np + node] = rho[m];
 }
 
 // Moments, forces, common velocity and collision of one node.
-// k_collide modes: pulled=1, apply_bc=1 for a step (pull from fin, Dirichlet
+// k_collide modes: pulled=1, apply_bc=1 for a step (pull from fin, face
 // BCs, collide into fout); pulled=0 for the first collision after init
 // (local read, no BCs); ex.f != nullptr exports the state (f, rho, u', F, j
 // in FP64) instead of colliding.
@@ -648,31 +843,40 @@ struct Export {
 };
 
 // rho after BCUpdateRho and j (DistributionCalcDensity/CalcFlux), forces
-// from the stencil density (FlowCalcForces, lbm_flow.F90:760-808) and the
-// common velocity (FlowUpdateUE, lbm_flow.F90:494-573)
-template <int D, int C, int NFF, bool FAST, class R, class A, int H>
+// from the stencil density (FlowCalcForces, lbm_flow.F90:760-808), the face
+// BCs (BCApply, with those forces) when bc is set, and the common velocity
+// (FlowUpdateUE, lbm_flow.F90:494-573).  rs is the node's stencil density:
+// the pre-BC sum of f, or the Dirichlet density.  A node whose faces are all
+// reflecting keeps that pre-BC density as rho (BCUpdateRho skips
+// reflecting faces, lbm_bc.F90:436-455).  BC = false leaves the face BCs
+// out of the code (kernels for domains without BC faces).
+template <int D, int C, int NFF, bool FAST, class R, bool BC = true, class A, int H>
 __device__ __forceinline__ void moments_forces(const Params<R>& p, const uint8_t* __restrict__ tag, uint32_t mk,
-                                               const NbrH<H>& nb, const R (&f)[C][Lat<D>::Q], const R (&rs)[C],
-                                               const A& acc, R (&rl)[C], R (&j)[C][3], R (&F)[C][3], R (&up)[3]) {
+                                               const NbrH<H>& nb, R (&f)[C][Lat<D>::Q], const R (&rs)[C],
+                                               const A& acc, R (&rl)[C], R (&j)[C][3], R (&F)[C][3], R (&up)[3],
+                                               bool bc, int node) {
   constexpr int Q = Lat<D>::Q;
+  auto sums = [&]() {
 #pragma unroll
-  for (int m = 0; m < C; m++) {
-    R s = (R)0;
+    for (int m = 0; m < C; m++) {
+      R s = (R)0;
 #pragma unroll
-    for (int n = 0; n < Q; n++) s = s + f[m][n];
-    rl[m] = s;
-    sfor<0, D>([&](auto DI) {
-      constexpr int d = decltype(DI)::value;
-      R a = (R)0;
-      sfor<0, Q>([&](auto NI) {
-        constexpr int n = decltype(NI)::value;
-        constexpr int c = cdir<D>(n, d);
-        if constexpr (c == 1) a = a + f[m][n];
-        if constexpr (c == -1) a = a - f[m][n];
+      for (int n = 0; n < Q; n++) s = s + f[m][n];
+      rl[m] = s;
+      sfor<0, D>([&](auto DI) {
+        constexpr int d = decltype(DI)::value;
+        R a = (R)0;
+        sfor<0, Q>([&](auto NI) {
+          constexpr int n = decltype(NI)::value;
+          constexpr int c = cdir<D>(n, d);
+          if constexpr (c == 1) a = a + f[m][n];
+          if constexpr (c == -1) a = a - f[m][n];
+        });
+        j[m][d] = a;
       });
-      j[m][d] = a;
-    });
-  }
+    }
+  };
+  if constexpr (!BC) sums();
 #pragma unroll
   for (int m = 0; m < C; m++) F[m][0] = F[m][1] = F[m][2] = (R)0;
   if ((p.flags & kFlagFS) && (mk & kFSNear)) fs_force<D, C, R, H>(p, tag, nb, rs, F);
@@ -684,6 +888,15 @@ __device__ __forceinline__ void moments_forces(const Params<R>& p, const uint8_t
   }
   if constexpr (NFF > 0) {
     if (p.flags & kFlagFF) ff_force<D, C, NFF, FAST, R>(p, acc, (mk & kFFClean) != 0, rs, F);
+  }
+  if constexpr (BC) {
+    const uint32_t faces = bc ? (mk >> kFaceShift) & 0x3fu : 0u;
+    if (faces) apply_bcs<D, C, R>(p, mk, node, f, F);
+    sums();
+    if (faces && !(faces & (uint32_t)p.bcupd)) {
+#pragma unroll
+      for (int m = 0; m < C; m++) rl[m] = rs[m];
+    }
   }
   R den = (R)0;
 #pragma unroll
@@ -919,13 +1132,12 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
     pull<D, C, S, R>(p, b, node, mk, nb, f, d0);
   else
     load_local<D, C, S, R>(p, b, node, f, d0);
-  if (apply_bc && (mk & (0x3fu << kFaceShift))) apply_dirichlet<D, C, R>(p, mk, f);
   R rs[C];
 #pragma unroll
   for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
   GlobalRho<R, 3> acc{b.rho, p.Np, nb};
   R rl[C], j[C][3], F[C][3], up[3];
-  moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
+  moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, apply_bc != 0, node);
   if (ex.f) {  // state export
 #pragma unroll
     for (int m = 0; m < C; m++) {
@@ -994,27 +1206,20 @@ __global__ void __launch_bounds__(256) k_sums(Params<R> p, Bufs<S, R> b, int pul
         for (int n = 0; n < L::Q; n++) s = s + f[m][n];
         rs[m] = s;
       }
-      if (pulled && (mk & (0x3fu << kFaceShift))) {
-#pragma unroll
-        for (int fc = 0; fc < 2 * D; fc++)
-          if (mk & (1u << (kFaceShift + fc)))
-#pragma unroll
-            for (int m = 0; m < C; m++) rs[m] = p.bcrho[fc][m];
-      }
+      if (pulled) dirichlet_rho<D, C, R>(p, mk, node, rs);
     } else {
 #pragma unroll
       for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
     }
-    if (pulled && (mk & (0x3fu << kFaceShift))) apply_dirichlet<D, C, R>(p, mk, f);
     GlobalRho<R, 3> acc{b.rho, p.Np, nb};
     R rl[C], j[C][3], F[C][3], up[3];
-    moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up);
+    moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, pulled != 0, node);
     double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
     double mass[C];
     incoming_mass<C, L::Q, R>(f, d0, mass);
 #pragma unroll
     for (int m = 0; m < C; m++) {
-      // the stored mass (the node's density after Dirichlet faces reset it)
+      // the stored mass (the node's density after the face BCs)
       v[m] += wide_rest<S>() ? mass[m] : (double)rl[m];
       rt += (double)p.mm[m] * (double)rl[m];
 #pragma unroll

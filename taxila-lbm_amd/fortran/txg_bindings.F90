@@ -6,6 +6,7 @@
 !   C g[m][n]          -> desc%g(n, m)          (value of -g_<m><n>)
 !   C gw[k][m]         -> desc%gw(m, k)         (value of -gw_<mineral k>_<component m>)
 !   C bc_density[f][m] -> desc%bc_density(m, f) (face f = 1..6: xm xp ym yp zm zp)
+!   C bc_data[f][k]    -> desc%bc_data(k, f)    (flow%bc_data(k, f) of FlowParseBC)
 module txg_bindings
   use iso_c_binding
   implicit none
@@ -15,7 +16,10 @@ module txg_bindings
        TXG_ERR_STATE = 3, TXG_ERR_UNSUPPORTED = 4, TXG_ERR_NOMEM = 5
   integer(c_int32_t), parameter :: TXG_SRT = 0, TXG_MRT = 1
   integer(c_int32_t), parameter :: TXG_FP64 = 0, TXG_FP32 = 1, TXG_FP32_FP16 = 2
-  integer(c_int32_t), parameter :: TXG_BC_NULL = 0, TXG_BC_DENSITY = 6
+  integer(c_int32_t), parameter :: TXG_BC_NULL = 0, TXG_BC_REFLECTING = 2, TXG_BC_VELOCITY = 5, &
+       TXG_BC_DENSITY = 6, TXG_BC_MOMENTUM = 7, TXG_BC_PRESSURE = 8, TXG_BC_PRESSURE_INLET = 9, &
+       TXG_BC_PRESSURE_OUTLET = 10, TXG_BC_FLUX = 11, TXG_BC_FLUX_INLET = 12, TXG_BC_FLUX_OUTLET = 13
+  integer, parameter :: TXG_MAX_BCDATA = 3 * TXG_MAXC
 
   type, bind(C) :: txg_desc
      integer(c_int32_t) :: ndims, ncomponents, nx, ny, nz
@@ -31,6 +35,7 @@ module txg_bindings
      real(c_double) :: gw(TXG_MAXC, TXG_MAXMIN)
      real(c_double) :: bc_density(TXG_MAXC, 6)
      real(c_double) :: rho0(TXG_MAXC)
+     real(c_double) :: bc_data(TXG_MAX_BCDATA, 6)
   end type txg_desc
 
   interface
@@ -56,6 +61,13 @@ module txg_bindings
        type(c_ptr), value :: h
        real(c_double), intent(in) :: rho(*), u(*)
      end function txg_set_state
+
+     integer(c_int) function txg_set_bc_values(h, face, vals) bind(C, name='txg_set_bc_values')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: face
+       real(c_double), intent(in) :: vals(*)
+     end function txg_set_bc_values
 
      integer(c_int) function txg_set_fi(h, fi) bind(C, name='txg_set_fi')
        import :: c_int, c_ptr, c_double

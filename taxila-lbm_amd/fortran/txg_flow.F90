@@ -169,24 +169,10 @@ contains
                flow%desc%gw(m, k), flag)
        end do
     end do
-    ! faces: periodic, or density (FlowParseBC, lbm_flow.F90:1031-1093)
+    ! face boundary conditions (FlowParseBC, lbm_flow.F90:1031-1309): the
+    ! option values go to bc_density / bc_data as flow%bc_data holds them
     do f = 1, 2 * flow%ndims
-       bname = trim(faces(f))
-       bval = .false.
-       call OptionsGetBool(opts, '-bc_density_' // trim(bname), bval, flag)
-       if (bval) then
-          flow%desc%bc_flags(f) = TXG_BC_DENSITY
-          if (flow%ncomponents == 1) then
-             flow%desc%bc_density(1, f) = -1.d0
-             call OptionsGetReal(opts, '-bc_density_' // trim(bname) // '_value', flow%desc%bc_density(1, f), flag)
-          else
-             do m = 1, flow%ncomponents
-                flow%desc%bc_density(m, f) = -1.d0
-                call OptionsGetReal(opts, '-bc_density_' // trim(bname) // '_' // trim(flow%names(m)), &
-                     flow%desc%bc_density(m, f), flag)
-             end do
-          end if
-       end if
+       call parse_face_bc(flow, opts, f, trim(faces(f)))
     end do
     ! outputs (lbm_flow.F90:267-279)
     call OptionsGetBool(opts, '-output_flow_fi', flow%io_fi, flag)
@@ -195,6 +181,113 @@ contains
     call OptionsGetBool(opts, '-output_rhot', flow%io_rhot, flag)
     call OptionsGetBool(opts, '-output_pressure', flow%io_prs, flag)
   end subroutine FlowSetFromOptions
+
+  ! FlowParseBC (lbm_flow.F90:1031-1309) for face f: at most one BC kind
+  ! per face; values as flow%bc_data(:, f)
+  subroutine parse_face_bc(flow, opts, f, bname)
+    type(flow_type), intent(inout) :: flow
+    type(options_type), intent(in) :: opts
+    integer, intent(in) :: f
+    character(len=*), intent(in) :: bname
+    logical :: bval, flag, done
+    integer :: m, cnt
+    done = .false.
+    if (flag_set('-bc_reflecting_' // bname)) call set_kind(TXG_BC_REFLECTING)
+    if (flag_set('-bc_density_' // bname)) then
+       call set_kind(TXG_BC_DENSITY)
+       if (flow%ncomponents == 1) then
+          flow%desc%bc_density(1, f) = -1.d0
+          call OptionsGetReal(opts, '-bc_density_' // bname // '_value', flow%desc%bc_density(1, f), flag)
+       else
+          do m = 1, flow%ncomponents
+             flow%desc%bc_density(m, f) = -1.d0
+             call OptionsGetReal(opts, '-bc_density_' // bname // '_' // trim(flow%names(m)), &
+                  flow%desc%bc_density(m, f), flag)
+          end do
+       end if
+    end if
+    if (flow%ncomponents == 1) then
+       if (flag_set('-bc_pressure_' // bname)) then
+          call set_kind(TXG_BC_PRESSURE)
+          flow%desc%bc_data(:, f) = -1.d0
+          call OptionsGetReal(opts, '-bc_pressure_' // bname // '_value', flow%desc%bc_data(1, f), flag)
+       end if
+       if (flag_set('-bc_flux_' // bname)) then
+          call set_kind(TXG_BC_FLUX)
+          flow%desc%bc_data(:, f) = 0.d0
+          call OptionsGetReal(opts, '-bc_flux_' // bname // '_value', flow%desc%bc_data(1, f), flag)
+       end if
+    end if
+    if (flag_set('-bc_pressure_inlet_' // bname)) then
+       call set_kind(TXG_BC_PRESSURE_INLET)
+       flow%desc%bc_data(:, f) = -1.d0
+       call OptionsGetReal(opts, '-bc_pressure_' // bname // '_value', flow%desc%bc_data(1, f), flag)
+       if (flow%ncomponents > 1) call OptionsGetReal(opts, '-bc_rho1_fraction_' // bname // '_value', &
+            flow%desc%bc_data(2, f), flag)
+    end if
+    if (flag_set('-bc_pressure_outlet_' // bname)) then
+       call set_kind(TXG_BC_PRESSURE_OUTLET)
+       flow%desc%bc_data(:, f) = -1.d0
+       call OptionsGetReal(opts, '-bc_pressure_' // bname // '_value', flow%desc%bc_data(1, f), flag)
+    end if
+    if (flag_set('-bc_momentum_' // bname)) then
+       call set_kind(TXG_BC_MOMENTUM)
+       flow%desc%bc_data(:, f) = -1.d0
+       do m = 1, flow%ncomponents
+          cnt = flow%ndims
+          if (flow%ncomponents == 1) then
+             call OptionsGetRealArray(opts, '-bc_momentum_' // bname // '_value', &
+                  flow%desc%bc_data(1:flow%ndims, f), cnt, flag)
+          else
+             call OptionsGetRealArray(opts, '-bc_momentum_' // bname // '_' // trim(flow%names(m)), &
+                  flow%desc%bc_data((m - 1) * flow%ndims + 1:m * flow%ndims, f), cnt, flag)
+          end if
+          if (flag .and. cnt /= flow%ndims) then
+             write(*, *) 'ERROR: Momentum BC on boundary ', bname, ' was provided the wrong number of components.'
+             error stop 1
+          end if
+       end do
+    end if
+    if (flag_set('-bc_flux_inlet_' // bname)) then
+       call set_kind(TXG_BC_FLUX_INLET)
+       flow%desc%bc_data(:, f) = -1.d0
+       call OptionsGetReal(opts, '-bc_flux_inlet_' // bname // '_value', flow%desc%bc_data(1, f), flag)
+       if (flow%ncomponents > 1) call OptionsGetReal(opts, '-bc_rho1_fraction_' // bname // '_value', &
+            flow%desc%bc_data(2, f), flag)
+    end if
+    if (flag_set('-bc_flux_outlet_' // bname)) then
+       call set_kind(TXG_BC_FLUX_OUTLET)
+       flow%desc%bc_data(:, f) = -1.d0
+       call OptionsGetReal(opts, '-bc_flux_outlet_' // bname // '_value', flow%desc%bc_data(1, f), flag)
+    end if
+    if (flag_set('-bc_velocity_' // bname)) then
+       call set_kind(TXG_BC_VELOCITY)
+       flow%desc%bc_data(:, f) = -1.d0
+       cnt = flow%ndims
+       call OptionsGetRealArray(opts, '-bc_velocity_' // bname // '_values', flow%desc%bc_data(1:flow%ndims, f), &
+            cnt, flag)
+       if (flag .and. cnt /= flow%ndims) then
+          write(*, *) 'ERROR: Velocity BC on boundary ', bname, ' was provided the wrong number of components.'
+          error stop 1
+       end if
+    end if
+  contains
+    logical function flag_set(key)
+      character(len=*), intent(in) :: key
+      bval = .false.
+      call OptionsGetBool(opts, key, bval, flag)
+      flag_set = bval
+    end function flag_set
+    subroutine set_kind(code)
+      integer(c_int32_t), intent(in) :: code
+      if (done .and. code /= TXG_BC_REFLECTING) then
+         write(*, *) 'ERROR: Multiple BCs provided for boundary ', bname
+         error stop 1
+      end if
+      done = .true.
+      flow%desc%bc_flags(f) = code
+    end subroutine set_kind
+  end subroutine parse_face_bc
 
   ! LBMSetUp's flow part (lbm.F90:140-175): device state for the domain
   subroutine FlowSetUp(flow)

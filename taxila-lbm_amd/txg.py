@@ -25,7 +25,11 @@ MAXMIN = 16
 
 SRT, MRT = 0, 1
 FP64, FP32, FP32_FP16 = 0, 1, 2
-BC_NULL, BC_DENSITY = 0, 6
+# boundary-condition codes (include/lbm_definitions.h:30-43)
+BC_NULL, BC_REFLECTING, BC_VELOCITY, BC_DENSITY, BC_MOMENTUM = 0, 2, 5, 6, 7
+BC_PRESSURE, BC_PRESSURE_INLET, BC_PRESSURE_OUTLET = 8, 9, 10
+BC_FLUX, BC_FLUX_INLET, BC_FLUX_OUTLET = 11, 12, 13
+MAX_BCDATA = 3 * 4
 FACES = ("xm", "xp", "ym", "yp", "zm", "zp")
 OK, ARG_ERROR, HIP_ERROR, STATE_ERROR, UNSUPPORTED_ERROR, NOMEM_ERROR = 0, 1, 2, 3, 4, 5
 PETSC_VEC_CLASSID = 1211214  # PetscBinaryIO.py:113
@@ -62,6 +66,7 @@ class Desc(C.Structure):
         ("gw", (C.c_double * MAXC) * MAXMIN),
         ("bc_density", (C.c_double * MAXC) * 6),
         ("rho0", C.c_double * MAXC),
+        ("bc_data", (C.c_double * MAX_BCDATA) * 6),
     ]
 
     @property
@@ -82,8 +87,10 @@ def make_desc(ndims=3, ncomponents=1, shape=(16, 16, 16), periodic=(1, 1, 1),
     lbm_relaxation.F90:58-80) overridden by the given values.
 
     g is a CxC nested list of -g_ij; gw a list (per mineral) of per-component
-    -gw values; bc a dict face -> list of per-component densities for
-    -bc_density_<face>."""
+    -gw values; bc a dict face -> either a list of per-component densities
+    (-bc_density_<face>) or a pair (BC_* code, option values) with the
+    values of the reference's flow%bc_data(:, face) (see TXG_BC_* in
+    include/taxila_lbm.h)."""
     d = Desc()
     d.ndims = ndims
     d.ncomponents = ncomponents
@@ -122,6 +129,14 @@ def make_desc(ndims=3, ncomponents=1, shape=(16, 16, 16), periodic=(1, 1, 1),
     if bc:
         for face, vals in bc.items():
             f = FACES.index(face)
+            if isinstance(vals, tuple) and vals[0] != BC_DENSITY:
+                code, data = vals
+                d.bc_flags[f] = int(code)
+                for k, v in enumerate(data):
+                    d.bc_data[f][k] = float(v)
+                continue
+            if isinstance(vals, tuple):
+                vals = vals[1]
             d.bc_flags[f] = BC_DENSITY
             for m, v in enumerate(vals):
                 d.bc_density[f][m] = float(v)
@@ -162,7 +177,7 @@ SYMBOLS = ("txg_desc_init", "txg_create", "txg_set_walls", "txg_set_state", "txg
            "txg_profile_read", "txg_tune", "txg_tune_info", "txg_batch_create", "txg_batch_add",
            "txg_batch_remove", "txg_batch_step", "txg_batch_size", "txg_batch_tune", "txg_batch_tune_info",
            "txg_batch_sync", "txg_batch_last_error", "txg_batch_destroy", "txg_flow_sums",
-           "txg_batch_flow_sums", "txg_get_mask")
+           "txg_batch_flow_sums", "txg_get_mask", "txg_set_bc_values")
 
 
 class TxgError(RuntimeError):
@@ -195,6 +210,7 @@ def lib():
         L.txg_tune_info.argtypes = [vp, vp, vp, vp, vp]
         L.txg_flow_sums.argtypes = [vp, vp]
         L.txg_get_mask.argtypes = [vp, vp]
+        L.txg_set_bc_values.argtypes = [vp, C.c_int32, vp]
         L.txg_batch_flow_sums.argtypes = [vp, vp, C.c_int32]
         L.txg_batch_create.argtypes = [C.POINTER(vp)]
         L.txg_batch_add.argtypes = [vp, vp]
@@ -214,6 +230,12 @@ def lib():
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def face_nodes(desc, face: int) -> int:
+    """Nodes on a face: xm/xp NY*NZ, ym/yp NX*NZ, zm/zp NX*NY."""
+    nz = desc.nz if desc.ndims == 3 else 1
+    return (desc.ny * nz, desc.nx * nz, desc.nx * desc.ny)[face // 2]
 
 
 class Domain:
@@ -261,12 +283,21 @@ class Domain:
         r = np.ascontiguousarray(rho, dtype=np.float64).ravel()
         assert r.size == self.N * self.C
         uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64).ravel()
+        assert uu is None or uu.size == self.N * self.D * self.C, "u is [(node*D + d)*C + m]"
         self._check(lib().txg_set_state(self.h, _p(r), _p(uu)), "txg_set_state")
 
     def set_fi(self, fi):
         f = np.ascontiguousarray(fi, dtype=np.float64).ravel()
         assert f.size == self.N * self.Q * self.C
         self._check(lib().txg_set_fi(self.h, _p(f)), "txg_set_fi")
+
+    def set_bc_values(self, face, vals):
+        """Per-node values of a face, vals[(i*D + d)*C + m] (txg_set_bc_values)."""
+        v = np.ascontiguousarray(vals, dtype=np.float64).ravel()
+        n = face_nodes(self.desc, face) * self.D * self.C
+        if v.size != n:
+            raise ValueError("face %d takes %d values, got %d" % (face, n, v.size))
+        self._check(lib().txg_set_bc_values(self.h, int(face), _p(v)), "txg_set_bc_values")
 
     def init(self):
         self._check(lib().txg_init(self.h), "txg_init")

@@ -1,0 +1,171 @@
+"""Face boundary conditions on the HIP step vs the FP64 oracle (GPU).
+
+Every BC form of FlowParseBC (lbm_flow.F90:1031-1309) that the step
+implements -- density, pressure, pressure inlet, one-component pressure
+outlet, momentum, flux, flux inlet, flux outlet, velocity, reflecting --
+on D3Q19 (two-pass and one-pass kernels) and D2Q9, with walls, forces and
+edges where faces of different BC types meet (BCApply's type order).  FP64
+must match the oracle bit for bit.  Per-node face values (txg_set_bc_values)
+go through the same comparison.
+"""
+import numpy as np
+import pytest
+
+import cases
+import oracle
+import txg
+
+pytestmark = pytest.mark.gpu
+
+B = txg
+
+CASES = [
+    # (name, desc kwargs, porous walls, steps)
+    ("d3_neumann_forms_c2", dict(ndims=3, ncomponents=2, shape=(22, 18, 14), periodic=(0, 0, 1), isotropy_order=4,
+                                 g=[[0, 0.1], [0.1, 0]], tau=[1.0, 0.8], gvt=[1e-4, -2e-4, 3e-4],
+                                 bc={"xm": (B.BC_MOMENTUM, [0.011, 0.002, -0.001, 0.004, -0.003, 0.002]),
+                                     "xp": (B.BC_FLUX_OUTLET, [0.006]),
+                                     "ym": (B.BC_FLUX_INLET, [0.008, 0.3]),
+                                     "yp": (B.BC_DENSITY, [0.9, 0.1])}), True, 15),
+    ("d3_velocity_pressure_c1", dict(ndims=3, ncomponents=1, shape=(26, 12, 10), periodic=(0, 0, 0),
+                                     isotropy_order=0, tau=[0.8],
+                                     bc={"xm": (B.BC_VELOCITY, [0.02, 0.001, -0.002]),
+                                         "xp": (B.BC_PRESSURE, [1.0 / 3.0]),
+                                         "zm": (B.BC_REFLECTING, []),
+                                         "zp": (B.BC_VELOCITY, [0.0, 0.003, 0.0])}), True, 20),
+    ("d3_reflecting_mrt_iso8", dict(ndims=3, ncomponents=2, shape=(20, 16, 18), periodic=(0, 1, 0),
+                                    isotropy_order=8, relaxation_mode=txg.MRT, g=[[0, 0.1], [0.1, 0]],
+                                    s_c=[1.0, 1 / 0.6], s_e=[1.19, 1.19], s_e2=[1.4, 1.4], s_q=[1.2, 1.2],
+                                    s_nu=[1.0, 1 / 0.6], gw=[[0.05, -0.05]],
+                                    bc={"xm": (B.BC_REFLECTING, []), "xp": (B.BC_DENSITY, [0.97, 0.03]),
+                                        "zm": (B.BC_REFLECTING, []), "zp": (B.BC_REFLECTING, [])}), True, 20),
+    ("d3_pressure_inlet_c2", dict(ndims=3, ncomponents=2, shape=(24, 14, 12), periodic=(0, 1, 0), isotropy_order=4,
+                                  g=[[0, 0.09], [0.11, 0]], tau=[1.0, 1.0],
+                                  bc={"xm": (B.BC_PRESSURE_INLET, [0.31, 0.8]),
+                                      "xp": (B.BC_FLUX_INLET, [0.004, 0.2])}), False, 20),
+    ("d3_flux_outlet_c1_edges", dict(ndims=3, ncomponents=1, shape=(18, 14, 12), periodic=(0, 0, 1),
+                                     isotropy_order=4, g=[[-0.02]], tau=[1.0], gvt=[0.0, 1e-5, 0.0],
+                                     bc={"xm": (B.BC_FLUX, [0.003]), "xp": (B.BC_PRESSURE_OUTLET, [1.0]),
+                                         "ym": (B.BC_VELOCITY, [0.001, 0.0, 0.0]),
+                                         "yp": (B.BC_REFLECTING, [])}), True, 20),
+    ("d2_mixed_c2", dict(ndims=2, ncomponents=2, shape=(40, 30), periodic=(0, 0), isotropy_order=8,
+                         g=[[0, 0.1], [0.1, 0]], tau=[1.0, 0.7], gvt=[2e-5, 0.0],
+                         bc={"xm": (B.BC_VELOCITY, [0.01, 0.001]), "xp": (B.BC_FLUX_OUTLET, [0.004]),
+                             "ym": (B.BC_REFLECTING, []), "yp": (B.BC_MOMENTUM, [0.001, 0.0, 0.0, -0.001])}),
+     True, 30),
+    ("d2_pressure_c1", dict(ndims=2, ncomponents=1, shape=(36, 20), periodic=(0, 0), isotropy_order=0,
+                            bc={"xm": (B.BC_PRESSURE, [1.003 / 3]), "xp": (B.BC_PRESSURE, [1.0 / 3]),
+                                "yp": (B.BC_REFLECTING, [])}), False, 40),
+]
+
+
+def build(kw, porous):
+    d = txg.make_desc(**kw)
+    shape = (d.nx, d.ny, d.nz) if d.ndims == 3 else (d.nx, d.ny)
+    walls = cases.random_walls(shape, 0.12, seed=11) if porous else np.zeros(d.nnodes)
+    if d.ncomponents == 2:
+        rho = cases.ball_ic(shape, min(shape) // 4, (0.1, 0.9), (0.9, 0.1))
+    else:
+        rho = 1.0 + 0.01 * np.random.default_rng(3).standard_normal(d.nnodes)
+    u = 0.005 * np.random.default_rng(5).standard_normal(d.nnodes * d.ndims * d.ncomponents)
+    return d, walls, rho, u
+
+
+def gpu(d, walls, rho, u, steps, kernel, vals=None):
+    dom = txg.Domain(d)
+    dom.tune(kernel, (0, 0), 0)
+    dom.set_walls(walls)
+    for f, v in (vals or {}).items():
+        dom.set_bc_values(f, v)
+    dom.set_state(rho, u)
+    dom.init()
+    dom.step(steps)
+    return dom
+
+
+def cpu(d, walls, rho, u, steps, vals=None):
+    o = oracle.Oracle(d)
+    o.set_walls(walls)
+    for f, v in (vals or {}).items():
+        o.set_bc_values(f, v)
+    o.set_state(rho, u)
+    o.init()
+    o.step(steps)
+    return o
+
+
+def assert_same(g, o, d, walls):
+    fl = walls == 0
+    C, D, Q = d.ncomponents, d.ndims, d.Q
+    fg, fo = g.get_fi(), o.get_fi()
+    m = np.repeat(fl, Q * C)
+    assert np.isfinite(fo[m]).all()
+    assert np.array_equal(fg[m], fo[m]), "fi max diff %g" % np.abs(fg[m] - fo[m]).max()
+    rg, ug, Fg = g.get_moments()
+    ro, uo, Fo = o.get_moments()
+    assert np.array_equal(rg[np.repeat(fl, C)], ro[np.repeat(fl, C)])
+    assert np.array_equal(ug[np.repeat(fl, C * D)], uo[np.repeat(fl, C * D)])
+    assert np.array_equal(Fg[np.repeat(fl, C * D)], Fo[np.repeat(fl, C * D)])
+
+
+@pytest.mark.parametrize("kernel", ["twopass", "onepass"])
+@pytest.mark.parametrize("name,kw,porous,steps", CASES, ids=[c[0] for c in CASES])
+def test_bc_fp64_bit_exact_vs_oracle(name, kw, porous, steps, kernel):
+    d, walls, rho, u = build(kw, porous)
+    if kernel == "onepass" and d.ndims == 2:
+        pytest.skip("the one-pass kernel is D3Q19 only")
+    print("case", name, kernel, flush=True)
+    assert_same(gpu(d, walls, rho, u, steps, kernel), cpu(d, walls, rho, u, steps), d, walls)
+
+
+@pytest.mark.parametrize("zseg", [1, 5, 0])
+def test_bc_onepass_segments(zseg):
+    """Faces cut by z-segments of every length march the same state."""
+    name, kw, porous, steps = CASES[0]
+    d, walls, rho, u = build(kw, porous)
+    dom = txg.Domain(d)
+    dom.tune("onepass", (64, 4), zseg)
+    dom.set_walls(walls)
+    dom.set_state(rho, u)
+    dom.init()
+    dom.step(10)
+    assert_same(dom, cpu(d, walls, rho, u, 10), d, walls)
+    with pytest.raises(txg.TxgError, match="64x4"):
+        dom.tune("onepass", (32, 8), 0)
+
+
+@pytest.mark.parametrize("kernel", ["twopass", "onepass"])
+def test_bc_values_per_node(kernel):
+    """Per-node momentum and density profiles (txg_set_bc_values)."""
+    kw = dict(ndims=3, ncomponents=2, shape=(20, 12, 10), periodic=(0, 0, 1), isotropy_order=4,
+              g=[[0, 0.1], [0.1, 0]], bc={"xm": (B.BC_MOMENTUM, [0.0] * 6), "xp": (B.BC_DENSITY, [0.9, 0.1])})
+    d, walls, rho, u = build(kw, True)
+    ny, nz, C, D = 12, 10, 2, 3
+    i = np.arange(ny * nz)
+    mom = np.zeros((ny * nz, D, C))
+    mom[:, 0, 0] = 0.01 * np.sin(np.pi * (i % ny + 0.5) / ny)
+    mom[:, 1, 1] = -0.002 * (i // ny) / nz
+    dens = np.zeros((ny * nz, D, C))
+    dens[:, 0, 0] = 0.9 + 0.01 * (i % ny) / ny
+    dens[:, 0, 1] = 0.1
+    vals = {0: mom.ravel(), 1: dens.ravel()}
+    assert_same(gpu(d, walls, rho, u, 15, kernel, vals), cpu(d, walls, rho, u, 15, vals), d, walls)
+
+
+def test_bc_batched_equals_alone():
+    """A member with BC faces steps the same in a batch as alone, next to a
+    periodic member (the batch switches to the face-BC kernel)."""
+    name, kw, porous, steps = CASES[0]
+    d, walls, rho, u = build(kw, porous)
+    kw2 = dict(ndims=3, ncomponents=2, shape=(16, 16, 16), isotropy_order=4, g=[[0, 0.1], [0.1, 0]])
+    d2, w2, r2, u2 = build(kw2, False)
+    alone = gpu(d, walls, rho, u, 12, "onepass").get_fi()
+    a = cases.run(txg.Domain, d2, w2, r2, u2, 0)
+    b = cases.run(txg.Domain, d, walls, rho, u, 0)
+    bt = txg.Batch()
+    bt.add(a)
+    bt.add(b)
+    bt.step(12)
+    assert np.array_equal(b.get_fi(), alone)
+    assert np.array_equal(a.get_fi(), gpu(d2, w2, r2, u2, 12, "onepass").get_fi())
+    bt.close()

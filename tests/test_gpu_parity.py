@@ -112,7 +112,7 @@ def test_fp64_bit_exact_vs_oracle(name, kw, wmode, icmode, steps, kernel):
 
 @pytest.mark.parametrize("tile,zseg", [((32, 8), 1), ((32, 8), 3), ((64, 4), 0), ((64, 4), 19), ((32, 8), 100)])
 def test_onepass_tiles_and_segments_fp64(tile, zseg):
-    name, kw, wm, icm, steps = CASES[[c[0] for c in CASES].index("d3_odd_mixed_bc")]
+    name, kw, wm, icm, steps = CASES[[c[0] for c in CASES].index("d3_odd_iso8_periodic")]
     d, walls, rho, u = build_case(name, kw, wm, icm)
     a = run_gpu(d, walls, rho, u, 12, "onepass", tile, zseg).get_fi()
     o = cases.run(oracle.Oracle, d, walls, rho, u, 12).get_fi()
@@ -141,8 +141,9 @@ def test_autotuner_keeps_state_and_results(prec):
     """The launch-shape autotuner times one-pass launches on the live state
     at the first one-pass step.  The state must not advance while it does,
     and the shape it picks must not change the arithmetic: an autotuned run
-    equals a run with a fixed shape bit for bit, in every precision."""
-    name, kw, wm, icm, steps = CASES[[c[0] for c in CASES].index("d3_porous_dirichlet")]
+    equals a run with a fixed shape bit for bit, in every precision.  (A
+    domain without BC faces, so that both tile shapes are candidates.)"""
+    name, kw, wm, icm, steps = CASES[[c[0] for c in CASES].index("d3_odd_iso8_periodic")]
     kw = dict(kw, precision=prec, rho0=[0.5, 0.5])
     d, walls, rho, u = build_case(name, kw, wm, icm)
     a = run_gpu(d, walls, rho, u, 0, "onepass")
