@@ -47,28 +47,6 @@ __global__ void __launch_bounds__(256) k_sums_final(const double* __restrict__ p
   if (threadIdx.x == 0) out[k] = sh[0];
 }
 
-// max |(old - new)/new| over node-components with new != 0; old := new
-__global__ void k_delta(double* __restrict__ old, const double* __restrict__ cur, int n, int first,
-                        double* __restrict__ part) {
-  __shared__ double sh[256];
-  double mx = 0.0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    double r = cur[i];
-    double o = first ? 0.0 : old[i];
-    double v = o + -1.0 * r;
-    double q = (r != 0.0) ? v / r : 0.0;
-    mx = fmax(mx, fabs(q));
-    old[i] = r;
-  }
-  sh[threadIdx.x] = mx;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
 template <class R>
 __global__ void k_fill_rho(R* __restrict__ rho, const double* __restrict__ src, const uint32_t* __restrict__ mask,
                            int N, int Np, int C) {
@@ -1254,16 +1232,21 @@ int txg_flow_sums(txg_handle* h, double* out) {
 
 int txg_delta_norm(txg_handle* h, double* norm) {
   if (!h || !norm) return TXG_ERR_ARG;
-  int rc = run_export(h);
+  if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
+  // the densities come from the populations in place (no state export);
+  // without a psi-gradient stencil the kernel forms them itself
+  int rc = h->d.isotropy_order != 0 ? ensure_rho(h) : TXG_OK;
   if (rc) return rc;
   const size_t n = (size_t)h->N * h->C;
   if (!h->rho_old && !h->alloc(&h->rho_old, n * 8)) return h->fail(TXG_ERR_NOMEM, "rho_old allocation");
-  // export rho is [m*N + node]; the reference Vec is node-major, but max is
+  // rho_old is [m*N + node]; the reference Vec is node-major, but max is
   // order independent
-  ExportPtrs e = export_ptrs(h);
-  const int nb = 1024;
-  hipLaunchKernelGGL(k_delta, dim3(nb), dim3(256), 0, h->stream, h->rho_old, e.rho, (int)n, h->have_old ? 0 : 1,
-                     h->red);
+  const int nb = kSumBlocks;
+  AnyBufs b = bufs(h, h->cur);
+  rc = h->hip(h->ks.delta(h->params(), b.ptr(), h->local_state ? 0 : 1, h->rho_old, h->have_old ? 0 : 1, h->red,
+                          h->stream),
+              "delta norm kernel");
+  if (rc) return rc;
   std::vector<double> part(nb);
   rc = h->download(part.data(), h->red, nb * 8, "delta download");
   if (rc) return rc;
@@ -1560,11 +1543,8 @@ int txg_batch_add(txg_batch* b, txg_handle* h) {
     b->bc = h->has_bc;
   } else if (h->C != b->C || iso != b->iso || h->prec != b->prec) {
     return b->fail(TXG_ERR_UNSUPPORTED, "batch members must share components, isotropy and precision");
-  } else if (h->has_bc && !b->bc) {
-    // a member with BC faces: the batch launches the variant with the face
-    // code from now on (it steps members without BC faces identically)
-    b->ks = h->ks;
-    b->bc = 1;
+  } else if (h->has_bc != b->bc) {
+    return b->fail(TXG_ERR_UNSUPPORTED, "members with and without boundary-condition faces cannot share a batch");
   }
   b->mem.push_back(h);
   h->batch = b;

@@ -44,6 +44,9 @@ struct KernelSet {
   // FP64 partial flow sums (k_sums): part[kSumBlocks * nsums]
   hipError_t (*sums)(const void* p, const void* b, int pulled, double* part, hipStream_t s);
   int nsums;
+  // steady-state delta norm of the density (k_delta_state): block maxima to
+  // part[kSumBlocks], old density [C*N] updated
+  hipError_t (*delta)(const void* p, const void* b, int pulled, double* old, int first, double* part, hipStream_t s);
   size_t member_bytes;  // sizeof(BatchMember<S,R>)
 };
 

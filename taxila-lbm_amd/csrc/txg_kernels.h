@@ -1176,6 +1176,46 @@ constexpr int num_sums() {
   return C + C * D + D + 1;
 }
 
+// the state of one node as the step leaves it: its populations after the
+// face BCs and its moments (rho after BCUpdateRho, j, forces, common
+// velocity); false for solid nodes.  pulled = 0 reads the local state
+// fresh from txg_init (no BCs).
+template <int D, int C, int NFF, bool FAST, class S, class R>
+__device__ __forceinline__ bool node_state(const Params<R>& p, const Bufs<S, R>& b, int pulled, int node,
+                                           R (&f)[C][Lat<D>::Q], R (&d0)[C], R (&rl)[C], R (&j)[C][3],
+                                           R (&F)[C][3], R (&up)[3]) {
+  using L = Lat<D>;
+  const uint32_t mk = b.mask[node];
+  if (mk & kSelfSolid) return false;
+  int x, y, z;
+  node_xyz(p, node, x, y, z);
+  Nbr nb;
+  make_nbr<D>(p, x, y, z, nb);
+  if (pulled)
+    pull<D, C, S, R>(p, b, node, mk, nb, f, d0);
+  else
+    load_local<D, C, S, R>(p, b, node, f, d0);
+  R rs[C];
+  if constexpr (NFF == 0) {
+    // no stencil: the node's own density (Dirichlet faces prescribe it),
+    // as the density pass would store it
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      R s = (R)0;
+#pragma unroll
+      for (int n = 0; n < L::Q; n++) s = s + f[m][n];
+      rs[m] = s;
+    }
+    if (pulled) dirichlet_rho<D, C, R>(p, mk, node, rs);
+  } else {
+#pragma unroll
+    for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
+  }
+  GlobalRho<R, 3> acc{b.rho, p.Np, nb};
+  moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, pulled != 0, node);
+  return true;
+}
+
 template <int D, int C, int NFF, bool FAST, class S, class R>
 __global__ void __launch_bounds__(256) k_sums(Params<R> p, Bufs<S, R> b, int pulled, double* __restrict__ part) {
   using L = Lat<D>;
@@ -1184,36 +1224,8 @@ __global__ void __launch_bounds__(256) k_sums(Params<R> p, Bufs<S, R> b, int pul
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = 0.0;
   for (int node = blockIdx.x * 256 + threadIdx.x; node < p.N; node += gridDim.x * 256) {
-    const uint32_t mk = b.mask[node];
-    if (mk & kSelfSolid) continue;
-    int x, y, z;
-    node_xyz(p, node, x, y, z);
-    Nbr nb;
-    make_nbr<D>(p, x, y, z, nb);
-    R f[C][L::Q], d0[C];
-    if (pulled)
-      pull<D, C, S, R>(p, b, node, mk, nb, f, d0);
-    else
-      load_local<D, C, S, R>(p, b, node, f, d0);
-    R rs[C];
-    if constexpr (NFF == 0) {
-      // no stencil: the node's own density (Dirichlet faces prescribe it),
-      // as the density pass would store it
-#pragma unroll
-      for (int m = 0; m < C; m++) {
-        R s = (R)0;
-#pragma unroll
-        for (int n = 0; n < L::Q; n++) s = s + f[m][n];
-        rs[m] = s;
-      }
-      if (pulled) dirichlet_rho<D, C, R>(p, mk, node, rs);
-    } else {
-#pragma unroll
-      for (int m = 0; m < C; m++) rs[m] = b.rho[(size_t)m * p.Np + node];
-    }
-    GlobalRho<R, 3> acc{b.rho, p.Np, nb};
-    R rl[C], j[C][3], F[C][3], up[3];
-    moments_forces<D, C, NFF, FAST, R>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, pulled != 0, node);
+    R f[C][L::Q], d0[C], rl[C], j[C][3], F[C][3], up[3];
+    if (!node_state<D, C, NFF, FAST, S, R>(p, b, pulled, node, f, d0, rl, j, F, up)) continue;
     double rt = 0.0, mom[3] = {0.0, 0.0, 0.0};
     double mass[C];
     incoming_mass<C, L::Q, R>(f, d0, mass);
@@ -1248,6 +1260,41 @@ __global__ void __launch_bounds__(256) k_sums(Params<R> p, Bufs<S, R> b, int pul
     const int k = threadIdx.x;
     part[(size_t)blockIdx.x * NV + k] = ((sm[0][k] + sm[1][k]) + sm[2][k]) + sm[3][k];
   }
+}
+
+// Steady-state delta norm from the populations in place
+// (DistributionCalcDeltaNorm, lbm_distribution_function.F90:809-833): each
+// block writes the max over its node-components with rho != 0 of
+// |(old - rho)/rho|, rho the node's density after the face BCs (the FP64
+// value txg_get_moments returns; 0 on solid nodes), and sets old := rho
+// ([m*N + node]).  first: old counts as zero.
+template <int D, int C, int NFF, bool FAST, class S, class R>
+__global__ void __launch_bounds__(256) k_delta_state(Params<R> p, Bufs<S, R> b, int pulled, double* __restrict__ old,
+                                                     int first, double* __restrict__ part) {
+  using L = Lat<D>;
+  double mx = 0.0;
+  for (int node = blockIdx.x * 256 + threadIdx.x; node < p.N; node += gridDim.x * 256) {
+    R f[C][L::Q], d0[C], rl[C], j[C][3], F[C][3], up[3];
+    const bool fluid = node_state<D, C, NFF, FAST, S, R>(p, b, pulled, node, f, d0, rl, j, F, up);
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      const double r = fluid ? (double)rl[m] : 0.0;
+      const size_t i = (size_t)m * p.N + node;
+      const double o = first ? 0.0 : old[i];
+      const double v = o + -1.0 * r;
+      const double q = (r != 0.0) ? v / r : 0.0;
+      mx = fmax(mx, fabs(q));
+      old[i] = r;
+    }
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 // FlowFiInit (lbm_flow.F90:923-934): f = (1 - pref/2) feq(rho_IC, u_IC) with

@@ -217,3 +217,24 @@ def test_reduced_precision_conserves_mass(prec, tol):
     dom.step(300)
     m1 = dom.flow_sums()["mass"]
     assert np.all(np.abs(m1 - m0) <= tol * m0), (m0, m1, (m1 - m0) / m0)
+
+
+@pytest.mark.parametrize("name,kernel", [("d3_porous_dirichlet", "onepass"), ("d3_porous_dirichlet", "twopass"),
+                                         ("d2_walls_fs_body", "twopass"), ("d3_single_phase_body", "onepass")])
+def test_delta_norm_matches_oracle(name, kernel):
+    """txg_delta_norm (computed on the GPU from the populations in place)
+    equals the oracle's DistributionCalcDeltaNorm bit for bit in FP64, from
+    the first call (rho_old = 0 gives 1) through later ones, including one
+    right after txg_init (the local state)."""
+    _, kw, wm, icm, _ = CASES[[c[0] for c in CASES].index(name)]
+    d, walls, rho, u = build_case(name, kw, wm, icm)
+    g = run_gpu(d, walls, rho, u, 0, kernel)
+    o = cases.run(oracle.Oracle, d, walls, rho, u, 0)
+    seq_g, seq_o = [g.delta_norm()], [o.delta_norm()]
+    for n in (1, 4, 7):
+        g.step(n)
+        o.step(n)
+        seq_g.append(g.delta_norm())
+        seq_o.append(o.delta_norm())
+    assert seq_g[0] == 1.0
+    assert seq_g == seq_o, (seq_g, seq_o)
