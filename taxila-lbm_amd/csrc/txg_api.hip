@@ -180,7 +180,8 @@ struct txg_handle {
   void* bcv = nullptr;               // R: per-node face values of the BC faces (Params::bcv)
   std::vector<double> hbcv[6];       // the same values in FP64, per face
   int bcoff[6] = {0, 0, 0, 0, 0, 0};
-  int has_bc = 0;                    // some face has a boundary condition
+  int has_bc = 0;                    // BC faces: 0 none, 1 uniform Dirichlet only, 2 any other
+  KernelSet ks0{};                   // the kernels as picked, before the BC-variant choice
   // FP64 export arrays
   double* ex = nullptr;
   size_t ex_len = 0;
@@ -590,6 +591,39 @@ int txg_desc_init(txg_desc* d) {
   return TXG_OK;
 }
 
+// The one-pass kernel variant a handle's faces need: none; Dirichlet faces
+// that each hold one density per component (its value goes to Params::bcrho,
+// the step kernel then reads no per-node values); or any other BC set-up.
+// Sets h->has_bc, the bcrho constants and the kernel arrays of h->ks.
+void select_bc_variant(txg_handle* h) {
+  const int C = h->C, D = h->D;
+  int v = 0;
+  for (int f = 0; f < 2 * D; f++) {
+    const int t = (h->p64.bckinds >> (3 * f)) & 7;
+    if (!t) continue;
+    bool uniform = t == kBcDirichlet;
+    const std::vector<double>& a = h->hbcv[f];
+    for (size_t i = 0; uniform && i < a.size(); i += (size_t)(D * C))
+      for (int m = 0; m < C; m++) uniform = uniform && a[i + m] == a[m];
+    if (uniform)
+      for (int m = 0; m < C; m++) {
+        h->p64.bcrho[f][m] = a[m];
+        h->p32.bcrho[f][m] = (float)a[m];
+      }
+    v = std::max(v, uniform ? 1 : 2);
+  }
+  h->has_bc = v;
+  h->ks = h->ks0;
+  if (v) {
+    for (int t = 0; t < kNumTiles; t++) {
+      h->ks.fused[t] = h->ks0.fused_bc[v - 1][t];
+      h->ks.fused_batch[t] = h->ks0.fused_batch_bc[v - 1][t];
+      h->ks.fused_batch_occupancy[t] = h->ks0.fused_batch_bc_occupancy[v - 1][t];
+    }
+    if (!h->ks.fused[h->tile]) h->tile = first_tile(h->ks.fused);
+  }
+}
+
 // copy face f's values (FP64 on the host) to the device in the compute type
 int upload_bc_values(txg_handle* h, int f) {
   const std::vector<double>& v = h->hbcv[f];
@@ -672,16 +706,7 @@ int txg_create(const txg_desc* d, txg_handle** out) {
   fill_params<float>(*d, h->N, h->Np, flags, dirichlet, h->p32);
   if (h->prec == TXG_FP32_FP16)
     for (int m = 0; m < h->C; m++) h->p32.shift0[m] = (float)d->rho0[m];
-  // domains with BC faces step with the kernel variants that carry the face code
-  for (int f = 0; f < 6; f++) h->has_bc |= bcs.kind[f] != 0;
-  if (h->has_bc) {
-    for (int t = 0; t < kNumTiles; t++) {
-      h->ks.fused[t] = h->ks.fused_bc[t];
-      h->ks.fused_batch[t] = h->ks.fused_batch_bc[t];
-      h->ks.fused_batch_occupancy[t] = h->ks.fused_batch_bc_occupancy[t];
-    }
-    h->tile = first_tile(h->ks.fused);
-  }
+  h->ks0 = h->ks;
   size_t bc_len = 0;
   for (int f = 0; f < 6; f++) {
     h->hbcv[f] = std::move(bcs.vals[f]);
@@ -690,6 +715,7 @@ int txg_create(const txg_desc* d, txg_handle** out) {
   }
   for (Params<double>* pp : {&h->p64}) bc_params(bcs, h->bcoff, *pp);
   for (Params<float>* pp : {&h->p32}) bc_params(bcs, h->bcoff, *pp);
+  select_bc_variant(h);
 
   // non-blocking: the handle's stream never waits for the legacy default
   // stream (nor it for the handle), so handles and batches overlap
@@ -758,7 +784,16 @@ int txg_set_bc_values(txg_handle* h, int32_t face, const double* vals) {
   if (!h || !vals) return TXG_ERR_ARG;
   if (face < 0 || face >= 2 * h->D) return h->fail(TXG_ERR_ARG, "face %d out of range", face);
   if (h->hbcv[face].empty()) return h->fail(TXG_ERR_ARG, "face %d has no boundary values", face);
+  const std::vector<double> keep = h->hbcv[face];
   std::copy(vals, vals + h->hbcv[face].size(), h->hbcv[face].begin());
+  const int before = h->has_bc;
+  select_bc_variant(h);
+  if (h->has_bc != before && h->batch) {
+    h->hbcv[face] = keep;
+    select_bc_variant(h);
+    return h->fail(TXG_ERR_STATE, "these values change the handle's kernel variant; remove it from its batch first");
+  }
+  if (h->has_bc != before) h->tuned = 0;  // the other variant's launch shape is timed anew
   return upload_bc_values(h, face);
 }
 
@@ -1310,7 +1345,7 @@ struct txg_batch {
   std::vector<txg_handle*> mem;
   KernelSet ks{};
   int C = 0, iso = -1, prec = -1;
-  int bc = 0;  // ks holds the kernel variants for boundary-condition faces
+  int bc = 0;  // the members' BC-face variant (txg_handle::has_bc)
   // device member table of the current launch set
   void* dmem = nullptr;
   size_t dmem_cap = 0;
@@ -1544,7 +1579,8 @@ int txg_batch_add(txg_batch* b, txg_handle* h) {
   } else if (h->C != b->C || iso != b->iso || h->prec != b->prec) {
     return b->fail(TXG_ERR_UNSUPPORTED, "batch members must share components, isotropy and precision");
   } else if (h->has_bc != b->bc) {
-    return b->fail(TXG_ERR_UNSUPPORTED, "members with and without boundary-condition faces cannot share a batch");
+    return b->fail(TXG_ERR_UNSUPPORTED, "members with and without boundary-condition faces (or with Dirichlet "
+                                        "faces only and with other BC types) cannot share a batch");
   }
   b->mem.push_back(h);
   h->batch = b;

@@ -35,12 +35,12 @@ struct KernelSet {
                                        hipStream_t s);
   // workgroups of the batched kernel resident per CU
   int (*fused_batch_occupancy[kNumTiles])();
-  // the same three for domains with boundary-condition faces (null where a
-  // tile shape has no such variant)
-  hipError_t (*fused_bc[kNumTiles])(const void* p, const void* b, const FusedGeom* g, hipStream_t s);
-  hipError_t (*fused_batch_bc[kNumTiles])(const void* mem, const int* prefix, int nmem, int flip, int grid,
-                                          hipStream_t s);
-  int (*fused_batch_bc_occupancy[kNumTiles])();
+  // the same three for domains with boundary-condition faces, [0] Dirichlet
+  // faces only, [1] any BC type (null where a tile shape has no variant)
+  hipError_t (*fused_bc[2][kNumTiles])(const void* p, const void* b, const FusedGeom* g, hipStream_t s);
+  hipError_t (*fused_batch_bc[2][kNumTiles])(const void* mem, const int* prefix, int nmem, int flip, int grid,
+                                             hipStream_t s);
+  int (*fused_batch_bc_occupancy[2][kNumTiles])();
   // FP64 partial flow sums (k_sums): part[kSumBlocks * nsums]
   hipError_t (*sums)(const void* p, const void* b, int pulled, double* part, hipStream_t s);
   int nsums;

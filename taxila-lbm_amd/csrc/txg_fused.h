@@ -236,7 +236,7 @@ struct FusedShape {
 // boundary is then where both segments start (or both end), so the halo
 // planes one segment reads beyond its range are read by the other at about
 // the same time, and the second read hits L2 instead of HBM.
-template <int C, int NFF, bool FAST, class S, class R, int TX, int TY, bool BC>
+template <int C, int NFF, bool FAST, class S, class R, int TX, int TY, int BCV>
 __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>& b, int x0, int y0, int z0, int z1,
                                             int dir, R* srho) {
   using L = Lat<3>;
@@ -349,7 +349,8 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
       for (int n = 0; n < Q; n++) s = s + f[m][n];
       rho[m] = s;
     }
-    if constexpr (BC)
+    if constexpr (BCV == kBcDirOnly) dirichlet_rho_uniform<3, C, R>(p, (mk >> kFaceShift) & 0x3fu, rho);
+    if constexpr (BCV == kBcAll)
       if (mk & (0x3fu << kFaceShift)) face_rho(mk, (int)((xr[1] + yr[1] + zr[1]) / SB), rho);
   };
 
@@ -407,7 +408,7 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
 #pragma unroll
     for (int m = 0; m < C; m++) rs[m] = acc.template get<0, 0, 0>(m);
     R rl[C], j[C][3], F[C][3], up[3];
-    moments_forces<3, C, NFF, FAST, R, BC>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, true, node);
+    moments_forces<3, C, NFF, FAST, R, BCV>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, true, node);
     R res[MAXC];
     pad_res<C, R>(d0, res);
     collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, b, rout, (uint32_t)node * SB, plane_b, res), f, rl, F, up);
@@ -438,13 +439,14 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
         for (int n = 0; n < Q; n++) s2 = s2 + f[m][n];
         rs[m] = s2;
       }
-      if constexpr (BC) dirichlet_rho<3, C, R>(p, mk, node, rs);
+      if constexpr (BCV == kBcDirOnly) dirichlet_rho_uniform<3, C, R>(p, (mk >> kFaceShift) & 0x3fu, rs);
+      if constexpr (BCV == kBcAll) dirichlet_rho<3, C, R>(p, mk, node, rs);
       LdsRho<R, C, 0, EXW, EYW> acc;  // unused without a stencil
       acc.s = srho;
       acc.col = 0;
       acc.slot[0] = 0;
       R rl[C], j[C][3], F[C][3], up[3];
-      moments_forces<3, C, NFF, FAST, R, BC>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, true, node);
+      moments_forces<3, C, NFF, FAST, R, BCV>(p, b.tag, mk, nb, f, rs, acc, rl, j, F, up, true, node);
       R res[MAXC];
       pad_res<C, R>(d0, res);
       collide_store<3, C, FAST, R>(p, BufStore<S, R>(p, b, rout, (uint32_t)node * SB, plane_b, res), f, rl, F, up);
@@ -536,9 +538,10 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
   }
 }
 
-// BC: the variant for domains with boundary-condition faces (the face code
-// costs registers, so domains without BC faces run the variant without it)
-template <int C, int NFF, bool FAST, class S, class R, int TX, int TY, bool BC>
+// BCV: the face-BC code compiled in (kBcNone, kBcDirOnly, kBcAll): the
+// face code costs registers and instruction cache, so each domain runs the
+// variant with only the code its faces need
+template <int C, int NFF, bool FAST, class S, class R, int TX, int TY, int BCV>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(FusedShape<NFF, TX, TY>::template waves<R>())))
 k_fused3(Params<R> p, Bufs<S, R> b, FusedGeom g) {
   __shared__ R srho[FusedShape<NFF, TX, TY>::template lds_len<C>()];
@@ -547,7 +550,7 @@ k_fused3(Params<R> p, Bufs<S, R> b, FusedGeom g) {
   const int lb = xcd_remap(blockIdx.x, g.nblocks);
   const int seg = lb % g.nseg, tile = lb / g.nseg;
   const int z0 = seg * g.zseg;
-  fused3_item<C, NFF, FAST, S, R, TX, TY, BC>(p, b, (tile % g.ntx) * TX, (tile / g.ntx) * TY, z0,
+  fused3_item<C, NFF, FAST, S, R, TX, TY, BCV>(p, b, (tile % g.ntx) * TX, (tile / g.ntx) * TY, z0,
                                               min(p.nz, z0 + g.zseg), (seg & 1) ? 1 : -1, srho);
 }
 
@@ -569,7 +572,7 @@ struct BatchMember {
 // the x-th eighth of the list with stride (workgroups on that XCD), so an
 // XCD's L2 serves contiguous tiles of the same member.  flip toggles every
 // member's orientation (every step flips all of them).
-template <int C, int NFF, bool FAST, class S, class R, int TX, int TY, bool BC>
+template <int C, int NFF, bool FAST, class S, class R, int TX, int TY, int BCV>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(FusedShape<NFF, TX, TY>::template waves<R>())))
 k_fused3_batch(const BatchMember<S, R>* __restrict__ mem, const int* __restrict__ prefix, int nmem, int flip) {
   __shared__ R srho[FusedShape<NFF, TX, TY>::template lds_len<C>()];
@@ -589,7 +592,7 @@ k_fused3_batch(const BatchMember<S, R>* __restrict__ mem, const int* __restrict_
     const int lb = it - prefix[a];
     const int seg = lb % m.g.nseg, tile = lb / m.g.nseg;
     const int z0 = seg * m.g.zseg;
-    fused3_item<C, NFF, FAST, S, R, TX, TY, BC>(m.p, m.b[m.parity ^ flip], (tile % m.g.ntx) * TX,
+    fused3_item<C, NFF, FAST, S, R, TX, TY, BCV>(m.p, m.b[m.parity ^ flip], (tile % m.g.ntx) * TX,
                                                 (tile / m.g.ntx) * TY, z0, min(m.p.nz, z0 + m.g.zseg),
                                                 (seg & 1) ? 1 : -1, srho);
   }

@@ -66,6 +66,7 @@ struct Params {
   const R* bcv;
   int bcoff[6];
   int bckinds, bcseq, nbcseq, bcupd;
+  R bcrho[6][MAXC];  // Dirichlet faces whose nodes all hold one density: that density
   R tau[MAXC];
   R mmot[MAXC];        // mm * s_c (lbm_flow.F90:518-520)
   R mm[MAXC];
@@ -345,17 +346,60 @@ __device__ __forceinline__ const R* face_vals(const Params<R>& p, int node) {
   return p.bcv + p.bcoff[F] + (size_t)face_index<F, R>(p, node) * (D * C);
 }
 
-// f_n += w_n sum_d c_n,d Q_d on the links that point inward from face F
-// (the last loop of the three node routines)
-template <int D, int C, int F, class R>
-__device__ __forceinline__ void add_incoming(R (&fm)[Lat<D>::Q], const R (&Q)[3]) {
+// The node corrections below take the face as a run-time value: one copy of
+// each routine serves all faces (six unrolled copies made the one-pass
+// kernel too large for the instruction cache).  A face's geometry: the
+// normal axis, the sign of the inward normal (DiscSetLocalDirections'
+// local_normal, lbm_discretization_d3q19.F90:566-714, _d2q9.F90:372-434:
+// + on xm/ym/zm) and the two in-face axes in CARDINAL_CROSS, _RESULTANT
+// order.
+template <int D>
+struct FaceGeo {
+  int nrm, cl, c1, c2;
+  __device__ __forceinline__ explicit FaceGeo(int F)
+      : nrm(F >> 1), cl((F & 1) ? -1 : 1), c1(D == 3 ? ((F >> 1) + 1) % 3 : 1 - (F >> 1)),
+        c2(D == 3 ? ((F >> 1) + 2) % 3 : 2) {}
+  // the value of an axis-indexed triple at the run-time axis a
+  template <class T>
+  __device__ __forceinline__ static T at(const T (&v)[3], int a) {
+    return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]);
+  }
+};
+
+// c_n along a run-time axis a (n a compile-time link)
+template <int D, int n>
+__device__ __forceinline__ int cax(int a) {
+  return a == 0 ? Lat<D>::cx[n] : (a == 1 ? Lat<D>::cy[n] : (D == 3 ? cdir<D>(n, 2) : 0));
+}
+
+// the face values vals(1:C, 1:D) of node `node` on run-time face F: the
+// value arrays of the faces that carry values (all but reflecting ones)
+// follow each other in face order
+template <int D, int C, class R, class P>
+__device__ __forceinline__ const R* face_vals_rt(const P& p, int node, int F) {
+  size_t off = 0;
+  sfor<0, 2 * D>([&](auto GI) {
+    constexpr int G = decltype(GI)::value;
+    const int t = (p.bckinds >> (3 * G)) & 7;
+    const size_t n = G / 2 == 0 ? (size_t)p.ny * p.nz : (G / 2 == 1 ? (size_t)p.nx * p.nz : (size_t)p.nxy);
+    if (G < F && t != 0 && t != kBcReflect) off += n * (D * C);
+  });
+  const int a = F >> 1;
+  int i;
+  if (a == 0) i = node / p.nx;
+  else if (a == 1) i = node % p.nx + p.nx * (node / p.nxy);
+  else i = node % p.nxy;
+  return p.bcv + off + (size_t)i * (D * C);
+}
+
+// f_n += w_n sum_d c_n,d Q_d on the links that point inward from the face
+// (the last loop of the three node routines); Q indexed by axis
+template <int D, class R>
+__device__ __forceinline__ void add_incoming(const FaceGeo<D>& g, R (&fm)[Lat<D>::Q], const R (&Q)[3]) {
   using L = Lat<D>;
-  constexpr int ln = FaceDirs<D>::ln[F];
-  constexpr int nrm = FaceDirs<D>::card[F][0];
-  constexpr int cl = cdir<D>(ln, nrm);
   sfor<1, L::Q>([&](auto NI) {
     constexpr int n = decltype(NI)::value;
-    if constexpr (cl * cdir<D>(n, nrm) == 1) {
+    if (g.cl * cax<D, n>(g.nrm) == 1) {
       R cq = (R)0;
       sfor<0, D>([&](auto DI) {
         constexpr int d = decltype(DI)::value;
@@ -368,105 +412,107 @@ __device__ __forceinline__ void add_incoming(R (&fm)[Lat<D>::Q], const R (&Q)[3]
   });
 }
 
-// momentum of fm along axis d (sum_n f_n c_n,d over n >= 1, zero terms
-// skipped) and the weights of the inward links with c_n,d != 0
-template <int D, int F, int d, class R>
-__device__ __forceinline__ void mom_wsum(const R (&fm)[Lat<D>::Q], R& mom, R& ws) {
+// momentum of fm along the run-time axis a (sum_n f_n c_n,a over n >= 1,
+// zero terms skipped) and the weights of the inward links with c_n,a != 0
+template <int D, class R>
+__device__ __forceinline__ void mom_wsum(const FaceGeo<D>& g, int a, const R (&fm)[Lat<D>::Q], R& mom, R& ws) {
   using L = Lat<D>;
-  constexpr int ln = FaceDirs<D>::ln[F];
-  constexpr int nrm = FaceDirs<D>::card[F][0];
-  constexpr int cl = cdir<D>(ln, nrm);
   sfor<1, L::Q>([&](auto NI) {
     constexpr int n = decltype(NI)::value;
-    constexpr int c = cdir<D>(n, d);
-    if constexpr (c == 1) mom = mom + fm[n];
-    if constexpr (c == -1) mom = mom - fm[n];
-    if constexpr (cl * cdir<D>(n, nrm) == 1 && c != 0) ws = ws + (R)L::w[n];
+    const int c = cax<D, n>(a);
+    if (c == 1) mom = mom + fm[n];
+    if (c == -1) mom = mom - fm[n];
+    if (g.cl * cax<D, n>(g.nrm) == 1 && c != 0) ws = ws + (R)L::w[n];
   });
 }
 
-// BCApplyDirichletNode (lbm_bc.F90:1273-1333) for face F: vals(m, 1) is the
-// density of component m
-template <int D, int C, int F, class R>
-__device__ __forceinline__ void dirichlet_node(const Params<R>& p, int node, R (&f)[C][Lat<D>::Q]) {
+// the Q triple by axis from the normal and in-face values
+template <int D, class R>
+__device__ __forceinline__ void q_by_axis(const FaceGeo<D>& g, R qn, R q1, R q2, R (&Q)[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) Q[d] = d == g.nrm ? qn : (d == g.c1 ? q1 : (d == g.c2 ? q2 : (R)0));
+}
+
+// BCApplyDirichletNode (lbm_bc.F90:1273-1333): vals(m, 1) is the density
+// of component m
+template <int D, int C, class R, class P>
+__device__ __forceinline__ void dirichlet_node(const P& p, int node, int F, R (&f)[C][Lat<D>::Q]) {
   using L = Lat<D>;
-  constexpr int ln = FaceDirs<D>::ln[F];
-  constexpr int nrm = FaceDirs<D>::card[F][0];
-  constexpr int cl = cdir<D>(ln, nrm);
-  const R* v = face_vals<D, C, F, R>(p, node);
+  const FaceGeo<D> g(F);
+  const R* v = face_vals_rt<D, C, R, P>(p, node, F);
 #pragma unroll
   for (int m = 0; m < C; m++) {
-    R wsum[3] = {(R)0, (R)0, (R)0}, Q[3] = {(R)0, (R)0, (R)0}, mom[3] = {(R)0, (R)0, (R)0};
+    R wn = (R)0;
     sfor<1, L::Q>([&](auto NI) {
       constexpr int n = decltype(NI)::value;
-      if constexpr (cl * cdir<D>(n, nrm) == 1) wsum[nrm] = wsum[nrm] + (R)L::w[n];
+      if (g.cl * cax<D, n>(g.nrm) == 1) wn = wn + (R)L::w[n];
     });
     R s = (R)0;
 #pragma unroll
     for (int n = 0; n < L::Q; n++) s = s + f[m][n];
-    Q[nrm] = (R)cl * (v[m] - s) / wsum[nrm];
-    sfor<1, D>([&](auto PI) {
-      constexpr int cp = FaceDirs<D>::card[F][decltype(PI)::value];
-      sfor<1, L::Q>([&](auto NI) {
-        constexpr int n = decltype(NI)::value;
-        constexpr int c = cdir<D>(n, cp);
-        if constexpr (c == 1) mom[cp] = mom[cp] + f[m][n];
-        if constexpr (c == -1) mom[cp] = mom[cp] - f[m][n];
-        if constexpr (cl * cdir<D>(n, nrm) == 1 && c != 0) wsum[cp] = wsum[cp] + (R)L::w[n];
-      });
-      Q[cp] = -mom[cp] / wsum[cp];
-    });
-    add_incoming<D, C, F, R>(f[m], Q);
-  }
-}
-
-// BCApplyNeumannNode (lbm_bc.F90:1533-1593) for face F: vals(m, d) is the
-// momentum j + F/2 the node carries after the correction
-template <int D, int C, int F, class R>
-__device__ __forceinline__ void neumann_node(const Params<R>& p, int node, R (&f)[C][Lat<D>::Q],
-                                             const R (&Fo)[C][3]) {
-  const R* v = face_vals<D, C, F, R>(p, node);
+    const R qn = (R)g.cl * (v[m] - s) / wn;
+    R qt[2] = {(R)0, (R)0};
 #pragma unroll
-  for (int m = 0; m < C; m++) {
-    R Q[3] = {(R)0, (R)0, (R)0};
-    sfor<0, D>([&](auto DI) {
-      constexpr int d = decltype(DI)::value;
+    for (int pi = 1; pi < D; pi++) {
       R mom = (R)0, ws = (R)0;
-      mom_wsum<D, F, d, R>(f[m], mom, ws);
-      Q[d] = (v[d * C + m] - Fo[m][d] / (R)2 - mom) / ws;
-    });
-    add_incoming<D, C, F, R>(f[m], Q);
+      mom_wsum<D, R>(g, pi == 1 ? g.c1 : g.c2, f[m], mom, ws);
+      qt[pi - 1] = -mom / ws;
+    }
+    R Q[3];
+    q_by_axis<D, R>(g, qn, qt[0], qt[1], Q);
+    add_incoming<D, R>(g, f[m], Q);
   }
 }
 
-// BCApplyVelocityNode (lbm_bc.F90:1793-1865) for face F: vals(1, d) is the
-// velocity of every component; the normal correction also sets the density
-template <int D, int C, int F, class R>
-__device__ __forceinline__ void velocity_node(const Params<R>& p, int node, R (&f)[C][Lat<D>::Q],
-                                              const R (&Fo)[C][3]) {
-  using L = Lat<D>;
-  constexpr int ln = FaceDirs<D>::ln[F];
-  constexpr int nrm = FaceDirs<D>::card[F][0];
-  constexpr int cl = cdir<D>(ln, nrm);
-  const R* v = face_vals<D, C, F, R>(p, node);
+// BCApplyNeumannNode (lbm_bc.F90:1533-1593): vals(m, d) is the momentum
+// j + F/2 the node carries after the correction
+template <int D, int C, class R, class P>
+__device__ __forceinline__ void neumann_node(const P& p, int node, int F, R (&f)[C][Lat<D>::Q],
+                                          const R (&Fo)[C][3]) {
+  const FaceGeo<D> g(F);
+  const R* v = face_vals_rt<D, C, R, P>(p, node, F);
 #pragma unroll
   for (int m = 0; m < C; m++) {
     R Q[3] = {(R)0, (R)0, (R)0};
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      R mom = (R)0, ws = (R)0;
+      mom_wsum<D, R>(g, d, f[m], mom, ws);
+      Q[d] = (v[d * C + m] - Fo[m][d] / (R)2 - mom) / ws;
+    }
+    add_incoming<D, R>(g, f[m], Q);
+  }
+}
+
+// BCApplyVelocityNode (lbm_bc.F90:1793-1865): vals(1, d) is the velocity of
+// every component; the normal correction also sets the density
+template <int D, int C, class R, class P>
+__device__ __forceinline__ void velocity_node(const P& p, int node, int F, R (&f)[C][Lat<D>::Q],
+                                           const R (&Fo)[C][3]) {
+  using L = Lat<D>;
+  const FaceGeo<D> g(F);
+  const R* v = face_vals_rt<D, C, R, P>(p, node, F);
+#pragma unroll
+  for (int m = 0; m < C; m++) {
     R mom = (R)0, ws = (R)0;
-    mom_wsum<D, F, nrm, R>(f[m], mom, ws);
+    mom_wsum<D, R>(g, g.nrm, f[m], mom, ws);
     R s = (R)0;
 #pragma unroll
     for (int n = 0; n < L::Q; n++) s = s + f[m][n];
-    const R un = v[nrm * C];
-    Q[nrm] = (s * un - mom - Fo[m][nrm] / (R)2) / ((R)1 - (R)cl * un) / ws;
-    const R rho = s + ws * Q[nrm] * (R)cl;
-    sfor<1, D>([&](auto PI) {
-      constexpr int cp = FaceDirs<D>::card[F][decltype(PI)::value];
+    const R un = v[g.nrm * C];
+    const R qn = (s * un - mom - FaceGeo<D>::at(Fo[m], g.nrm) / (R)2) / ((R)1 - (R)g.cl * un) / ws;
+    const R rho = s + ws * qn * (R)g.cl;
+    R qt[2] = {(R)0, (R)0};
+#pragma unroll
+    for (int pi = 1; pi < D; pi++) {
+      const int cp = pi == 1 ? g.c1 : g.c2;
       R mp = (R)0, wp = (R)0;
-      mom_wsum<D, F, cp, R>(f[m], mp, wp);
-      Q[cp] = (rho * v[cp * C] - Fo[m][cp] / (R)2 - mp) / wp;
-    });
-    add_incoming<D, C, F, R>(f[m], Q);
+      mom_wsum<D, R>(g, cp, f[m], mp, wp);
+      qt[pi - 1] = (rho * v[cp * C] - FaceGeo<D>::at(Fo[m], cp) / (R)2 - mp) / wp;
+    }
+    R Q[3];
+    q_by_axis<D, R>(g, qn, qt[0], qt[1], Q);
+    add_incoming<D, R>(g, f[m], Q);
   }
 }
 
@@ -512,22 +558,89 @@ __device__ __forceinline__ void reflect_node(R (&f)[C][Lat<D>::Q]) {
 // the order the faces first name the types, then xm..zp); Fo are the
 // node's forces (FlowCalcForces) the Neumann and velocity forms use
 template <int D, int C, class R>
-__device__ __forceinline__ void apply_bcs(const Params<R>& p, uint32_t mk, int node, R (&f)[C][Lat<D>::Q],
+__device__ __forceinline__ void apply_bcs(const Params<R>& p, uint32_t faces, int node, R (&f)[C][Lat<D>::Q],
                                           const R (&Fo)[C][3]) {
-  const uint32_t faces = (mk >> kFaceShift) & 0x3fu;
   for (int k = 0; k < p.nbcseq; k++) {
     const int fk = (p.bcseq >> (3 * k)) & 7;
     if (!((faces >> fk) & 1u)) continue;
-    sfor<0, 2 * D>([&](auto FI) {
-      constexpr int F = decltype(FI)::value;
-      if (F != fk) return;
-      const int t = (p.bckinds >> (3 * F)) & 7;
-      if (t == kBcDirichlet) dirichlet_node<D, C, F, R>(p, node, f);
-      else if (t == kBcNeumann) neumann_node<D, C, F, R>(p, node, f, Fo);
-      else if (t == kBcVelocity) velocity_node<D, C, F, R>(p, node, f, Fo);
-      else if (t == kBcReflect) reflect_node<D, C, F, R>(f);
+    const int t = (p.bckinds >> (3 * fk)) & 7;
+    if (t == kBcDirichlet) dirichlet_node<D, C, R>(p, node, fk, f);
+    else if (t == kBcNeumann) neumann_node<D, C, R>(p, node, fk, f, Fo);
+    else if (t == kBcVelocity) velocity_node<D, C, R>(p, node, fk, f, Fo);
+    else if (t == kBcReflect)
+      sfor<0, 2 * D>([&](auto FI) {
+        constexpr int F = decltype(FI)::value;
+        if (F == fk) reflect_node<D, C, F, R>(f);
+      });
+  }
+}
+
+// BCApplyDirichletNode (lbm_bc.F90:1273-1333) for face F with one density
+// per component on the whole face (p.bcrho): the form of the kernels for
+// domains whose BC faces are all such Dirichlet faces (the common case,
+// kept as small as the step kernel without faces)
+template <int D, int C, int F, class R>
+__device__ __forceinline__ void dirichlet_node_uniform(const Params<R>& p, R (&f)[C][Lat<D>::Q]) {
+  using L = Lat<D>;
+  constexpr int ln = FaceDirs<D>::ln[F];
+  constexpr int nrm = FaceDirs<D>::card[F][0];
+  constexpr int cl = cdir<D>(ln, nrm);
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R wsum[3] = {(R)0, (R)0, (R)0}, Q[3] = {(R)0, (R)0, (R)0}, mom[3] = {(R)0, (R)0, (R)0};
+    sfor<1, L::Q>([&](auto NI) {
+      constexpr int n = decltype(NI)::value;
+      if constexpr (cl * cdir<D>(n, nrm) == 1) wsum[nrm] = wsum[nrm] + (R)L::w[n];
+    });
+    R s = (R)0;
+#pragma unroll
+    for (int n = 0; n < L::Q; n++) s = s + f[m][n];
+    Q[nrm] = (R)cl * (p.bcrho[F][m] - s) / wsum[nrm];
+    sfor<1, D>([&](auto PI) {
+      constexpr int cp = FaceDirs<D>::card[F][decltype(PI)::value];
+      sfor<1, L::Q>([&](auto NI) {
+        constexpr int n = decltype(NI)::value;
+        constexpr int c = cdir<D>(n, cp);
+        if constexpr (c == 1) mom[cp] = mom[cp] + f[m][n];
+        if constexpr (c == -1) mom[cp] = mom[cp] - f[m][n];
+        if constexpr (cl * cdir<D>(n, nrm) == 1 && c != 0) wsum[cp] = wsum[cp] + (R)L::w[n];
+      });
+      Q[cp] = -mom[cp] / wsum[cp];
+    });
+    sfor<1, L::Q>([&](auto NI) {
+      constexpr int n = decltype(NI)::value;
+      if constexpr (cl * cdir<D>(n, nrm) == 1) {
+        R cq = (R)0;
+        sfor<0, D>([&](auto DI) {
+          constexpr int d = decltype(DI)::value;
+          constexpr int c = cdir<D>(n, d);
+          if constexpr (c == 1) cq = cq + Q[d];
+          if constexpr (c == -1) cq = cq - Q[d];
+        });
+        f[m][n] = f[m][n] + (R)L::w[n] * cq;
+      }
     });
   }
+}
+
+// The Dirichlet faces of a node, xm..zp (BCApplyDirichletD3 visits its
+// faces in that order), all with uniform densities
+template <int D, int C, class R>
+__device__ __forceinline__ void apply_dirichlet(const Params<R>& p, uint32_t faces, R (&f)[C][Lat<D>::Q]) {
+  sfor<0, 2 * D>([&](auto FI) {
+    constexpr int F = decltype(FI)::value;
+    if ((faces >> F) & 1u) dirichlet_node_uniform<D, C, F, R>(p, f);
+  });
+}
+
+// the stencil density of a node on Dirichlet faces with uniform densities
+template <int D, int C, class R>
+__device__ __forceinline__ void dirichlet_rho_uniform(const Params<R>& p, uint32_t faces, R (&rho)[C]) {
+#pragma unroll
+  for (int fc = 0; fc < 2 * D; fc++)
+    if ((faces >> fc) & 1u)
+#pragma unroll
+      for (int m = 0; m < C; m++) rho[m] = p.bcrho[fc][m];
 }
 
 // the density the gradient stencil sees on a node of Dirichlet faces
@@ -546,32 +659,33 @@ __device__ __forceinline__ void dirichlet_rho(const Params<R>& p, uint32_t mk, i
   });
 }
 
-// The same with the Dirichlet faces' value arrays taken out of Params once
-// per work item (the one-pass kernel's density pass over ring columns: a
-// Params access there made the compiler copy the kernel's parameter block
-// to scratch memory)
+// The same with what it needs taken out of Params once per work item (the
+// one-pass kernel's density pass over ring columns: a Params access there
+// made the compiler copy the kernel's parameter block to scratch memory),
+// and a run-time loop over the faces, which keeps the code that every
+// inlined column pull carries small.
 template <int D, int C, class R>
 struct FaceRho {
-  const R* v[2 * D];  // face F's values (null unless Dirichlet)
-  int dmask, nx, nxy;
-  __device__ __forceinline__ explicit FaceRho(const Params<R>& p) : dmask(p.dirichlet), nx(p.nx), nxy(p.nxy) {
-    sfor<0, 2 * D>([&](auto FI) {
-      constexpr int F = decltype(FI)::value;
-      v[F] = ((p.dirichlet >> F) & 1) ? p.bcv + p.bcoff[F] : nullptr;
-    });
-  }
+  const R* bcv;
+  int dmask, kinds, nx, ny, nz, nxy;
+  __device__ __forceinline__ explicit FaceRho(const Params<R>& p)
+      : bcv(p.bcv), dmask(p.dirichlet), kinds(p.bckinds), nx(p.nx), ny(p.ny), nz(p.nz), nxy(p.nxy) {}
   __device__ __forceinline__ void operator()(uint32_t mk, int node, R (&rho)[C]) const {
     const uint32_t faces = (mk >> kFaceShift) & (uint32_t)dmask;
     if (!faces) return;
-    sfor<0, 2 * D>([&](auto FI) {
-      constexpr int F = decltype(FI)::value;
+    size_t off = 0;  // start of face F's values
+#pragma unroll 1
+    for (int F = 0; F < 2 * D; F++) {
+      const int a = F >> 1;
       if ((faces >> F) & 1u) {
-        const int i = F / 2 == 0 ? node / nx : (F / 2 == 1 ? node % nx + nx * (node / nxy) : node % nxy);
-        const R* w = v[F] + (size_t)i * (D * C);
+        const int i = a == 0 ? node / nx : (a == 1 ? node % nx + nx * (node / nxy) : node % nxy);
+        const R* w = bcv + off + (size_t)i * (D * C);
 #pragma unroll
         for (int m = 0; m < C; m++) rho[m] = w[m];
       }
-    });
+      const int t = (kinds >> (3 * F)) & 7;
+      if (t != 0 && t != kBcReflect) off += (size_t)(a == 0 ? ny * nz : (a == 1 ? nx * nz : nxy)) * (D * C);
+    }
   }
 };
 
@@ -842,15 +956,22 @@ struct Export {
   double* j;     // [(m*D+d)*N + node]
 };
 
+// Face-BC code in a kernel: none (domains without BC faces), Dirichlet faces
+// with one density per face only, or every BC type with per-node values.
+// Each variant is a separate instantiation of the one-pass kernels, so a
+// domain carries only the face code it needs.
+constexpr int kBcNone = 0, kBcDirOnly = 1, kBcAll = 2;
+
 // rho after BCUpdateRho and j (DistributionCalcDensity/CalcFlux), forces
 // from the stencil density (FlowCalcForces, lbm_flow.F90:760-808), the face
 // BCs (BCApply, with those forces) when bc is set, and the common velocity
 // (FlowUpdateUE, lbm_flow.F90:494-573).  rs is the node's stencil density:
 // the pre-BC sum of f, or the Dirichlet density.  A node whose faces are all
 // reflecting keeps that pre-BC density as rho (BCUpdateRho skips
-// reflecting faces, lbm_bc.F90:436-455).  BC = false leaves the face BCs
-// out of the code (kernels for domains without BC faces).
-template <int D, int C, int NFF, bool FAST, class R, bool BC = true, class A, int H>
+// reflecting faces, lbm_bc.F90:436-455).  BCV selects the face code
+// compiled in (kBcNone, kBcDirOnly, kBcAll); the Dirichlet correction does
+// not read the forces, so with Dirichlet faces only it runs first.
+template <int D, int C, int NFF, bool FAST, class R, int BCV = kBcAll, class A, int H>
 __device__ __forceinline__ void moments_forces(const Params<R>& p, const uint8_t* __restrict__ tag, uint32_t mk,
                                                const NbrH<H>& nb, R (&f)[C][Lat<D>::Q], const R (&rs)[C],
                                                const A& acc, R (&rl)[C], R (&j)[C][3], R (&F)[C][3], R (&up)[3],
@@ -876,7 +997,10 @@ __device__ __forceinline__ void moments_forces(const Params<R>& p, const uint8_t
       });
     }
   };
-  if constexpr (!BC) sums();
+  const uint32_t faces = (BCV != kBcNone && bc) ? (mk >> kFaceShift) & 0x3fu : 0u;
+  if constexpr (BCV == kBcDirOnly)
+    if (faces) apply_dirichlet<D, C, R>(p, faces, f);
+  if constexpr (BCV != kBcAll) sums();
 #pragma unroll
   for (int m = 0; m < C; m++) F[m][0] = F[m][1] = F[m][2] = (R)0;
   if ((p.flags & kFlagFS) && (mk & kFSNear)) fs_force<D, C, R, H>(p, tag, nb, rs, F);
@@ -889,9 +1013,8 @@ __device__ __forceinline__ void moments_forces(const Params<R>& p, const uint8_t
   if constexpr (NFF > 0) {
     if (p.flags & kFlagFF) ff_force<D, C, NFF, FAST, R>(p, acc, (mk & kFFClean) != 0, rs, F);
   }
-  if constexpr (BC) {
-    const uint32_t faces = bc ? (mk >> kFaceShift) & 0x3fu : 0u;
-    if (faces) apply_bcs<D, C, R>(p, mk, node, f, F);
+  if constexpr (BCV == kBcAll) {
+    if (faces) apply_bcs<D, C, R>(p, faces, node, f, F);
     sums();
     if (faces && !(faces & (uint32_t)p.bcupd)) {
 #pragma unroll
