@@ -1202,46 +1202,27 @@ int txg_get_moments(txg_handle* h, double* rho, double* flux, double* forces) {
 
 int txg_diagnostics(txg_handle* h, double* rhot, double* prs, double* velt) {
   if (!h) return TXG_ERR_ARG;
-  int rc = run_export(h);
+  if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
+  // computed on the GPU from the populations in place (no state export);
+  // without a psi-gradient stencil the kernel forms the densities itself
+  int rc = h->d.isotropy_order != 0 ? ensure_rho(h) : TXG_OK;
   if (rc) return rc;
-  const size_t N = h->N, C = h->C, D = h->D;
-  ExportPtrs e = export_ptrs(h);
-  std::vector<double> r(N * C), F(N * C * D), j(N * C * D);
-  rc = h->download(r.data(), e.rho, r.size() * 8, "rho download");
-  if (!rc) rc = h->download(F.data(), e.F, F.size() * 8, "F download");
-  if (!rc) rc = h->download(j.data(), e.j, j.size() * 8, "j download");
-  if (rc) return rc;
-  const txg_desc& d = h->d;
-  // FlowUpdateDiagnosticsD3/D2 (lbm_flow.F90:654-758), psi = rho
-  for (size_t i = 0; i < N; i++) {
-    if (h->hsolid[i]) {
-      if (rhot) rhot[i] = 0.0;
-      if (prs) prs[i] = 0.0;
-      if (velt)
-        for (size_t dd = 0; dd < D; dd++) velt[i * D + dd] = 0.0;
-      continue;
-    }
-    double rt = 0.0;
-    for (size_t m = 0; m < C; m++) rt = rt + r[m * N + i] * d.mm[m];
-    double p = rt / 3.;
-    if (C > 1)
-      for (size_t m = 0; m < C; m++) {
-        double s = 0.0;
-        for (size_t mp = 0; mp < C; mp++) s = s + d.g[m][mp] * r[mp * N + i];
-        p = p + 6.0 / 2. * r[m * N + i] * s;
-      }
-    if (rhot) rhot[i] = rt;
-    if (prs) prs[i] = p;
-    for (size_t dd = 0; dd < D; dd++) {
-      double s = 0.0;
-      for (size_t m = 0; m < C; m++) {
-        double u = j[(m * D + dd) * N + i] + .5 * F[(m * D + dd) * N + i];
-        s = s + u * d.mm[m];
-      }
-      if (velt) velt[i * D + dd] = s / rt;
-    }
+  const size_t N = h->N, D = h->D;
+  double* dv = nullptr;
+  if (!h->alloc(&dv, N * (2 + D) * sizeof(double))) return h->fail(TXG_ERR_NOMEM, "diagnostics allocation");
+  DiagConst dc{};
+  for (int m = 0; m < h->C; m++) {
+    dc.mm[m] = h->d.mm[m];
+    for (int n = 0; n < h->C; n++) dc.g[m][n] = h->d.g[m][n];
   }
-  return TXG_OK;
+  AnyBufs b = bufs(h, h->cur);
+  rc = h->hip(h->ks.diag(h->params(), b.ptr(), h->local_state ? 0 : 1, &dc, dv, dv + N, dv + 2 * N, h->N, h->stream),
+              "diagnostics kernel");
+  if (!rc && rhot) rc = h->download(rhot, dv, N * 8, "rhot download");
+  if (!rc && prs) rc = h->download(prs, dv + N, N * 8, "prs download");
+  if (!rc && velt) rc = h->download(velt, dv + 2 * N, N * D * 8, "velt download");
+  h->release(dv);
+  return rc;
 }
 
 int txg_reduce(txg_handle* h, double* out) {

@@ -1420,6 +1420,60 @@ __global__ void __launch_bounds__(256) k_delta_state(Params<R> p, Bufs<S, R> b, 
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// FlowUpdateDiagnosticsD3/D2 (lbm_flow.F90:654-758, psi = rho) from the
+// populations in place: total density, pressure and total velocity per
+// node, in FP64 from the node's post-BC moments, with the options' mm and
+// g in FP64 (walls: 0, flow%null_pressure)
+struct DiagConst {
+  double mm[MAXC];
+  double g[MAXC][MAXC];
+};
+
+template <int D, int C, int NFF, bool FAST, class S, class R>
+__global__ void __launch_bounds__(256) k_diag(Params<R> p, Bufs<S, R> b, int pulled, DiagConst dc,
+                                              double* __restrict__ rhot, double* __restrict__ prs,
+                                              double* __restrict__ velt) {
+  using L = Lat<D>;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= p.N) return;
+  R f[C][L::Q], d0[C], rl[C], j[C][3], F[C][3], up[3];
+  if (!node_state<D, C, NFF, FAST, S, R>(p, b, pulled, node, f, d0, rl, j, F, up)) {
+    rhot[node] = 0.0;
+    prs[node] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) velt[(size_t)node * D + d] = 0.0;
+    return;
+  }
+  double r[C];
+#pragma unroll
+  for (int m = 0; m < C; m++) r[m] = (double)rl[m];
+  double rt = 0.0;
+#pragma unroll
+  for (int m = 0; m < C; m++) rt = rt + r[m] * dc.mm[m];
+  double pr = rt / 3.;
+  if constexpr (C > 1) {
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      double sg = 0.0;
+#pragma unroll
+      for (int mp = 0; mp < C; mp++) sg = sg + dc.g[m][mp] * r[mp];
+      pr = pr + 6.0 / 2. * r[m] * sg;
+    }
+  }
+  rhot[node] = rt;
+  prs[node] = pr;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    double sv = 0.0;
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      const double u = (double)j[m][d] + .5 * (double)F[m][d];
+      sv = sv + u * dc.mm[m];
+    }
+    velt[(size_t)node * D + d] = sv / rt;
+  }
+}
+
 // FlowFiInit (lbm_flow.F90:923-934): f = (1 - pref/2) feq(rho_IC, u_IC) with
 // forces from the IC density; rho[] holds the IC density (NaN on solids).
 template <int D, int C, int NFF, class S, class R>

@@ -238,3 +238,48 @@ def test_delta_norm_matches_oracle(name, kernel):
         seq_o.append(o.delta_norm())
     assert seq_g[0] == 1.0
     assert seq_g == seq_o, (seq_g, seq_o)
+
+
+@pytest.mark.parametrize("name,kernel", [("d3_porous_dirichlet", "onepass"), ("d2_walls_fs_body", "twopass"),
+                                         ("d3_single_phase_body", "twopass")])
+def test_diagnostics_match_oracle(name, kernel):
+    """txg_diagnostics (total density, pressure, total velocity computed on
+    the GPU from the populations in place) equals the oracle's
+    FlowUpdateDiagnostics bit for bit in FP64, walls included (zeros), right
+    after txg_init and after steps."""
+    _, kw, wm, icm, _ = CASES[[c[0] for c in CASES].index(name)]
+    d, walls, rho, u = build_case(name, kw, wm, icm)
+    g = run_gpu(d, walls, rho, u, 0, kernel)
+    o = cases.run(oracle.Oracle, d, walls, rho, u, 0)
+    for n in (0, 5):
+        if n:
+            g.step(n)
+            o.step(n)
+        for a, b in zip(g.diagnostics(), o.diagnostics()):
+            assert np.array_equal(a, b), np.abs(a - b).max()
+
+
+@pytest.mark.parametrize("name,kernel", [("d3_porous_dirichlet", "onepass"), ("d2_walls_fs_body", "twopass"),
+                                         ("d3_odd_iso8_periodic", "onepass")])
+def test_restart_from_fi_matches_oracle(name, kernel):
+    """Restart (LBMInitializeStateRestarted, lbm.F90:518-541, then LBMInit2
+    without FlowFiInit): populations written at some step are loaded with
+    txg_set_fi, and the run continues exactly as the oracle's does from the
+    same file (FP64, bit for bit)."""
+    _, kw, wm, icm, _ = CASES[[c[0] for c in CASES].index(name)]
+    d, walls, rho, u = build_case(name, kw, wm, icm)
+    fi = cases.run(oracle.Oracle, d, walls, rho, u, 7).get_fi()
+    g = txg.Domain(d)
+    g.tune(kernel, (0, 0), 0)
+    g.set_walls(walls)
+    g.set_fi(fi)
+    g.init()
+    g.step(6)
+    o = cases.run(oracle.Oracle, d, walls, None, None, 6, fi=fi)
+    m = np.repeat(walls == 0, d.Q * d.ncomponents)
+    assert np.array_equal(g.get_fi()[m], o.get_fi()[m])
+    fl = walls == 0
+    rg, ug, Fg = g.get_moments()
+    ro, uo, Fo = o.get_moments()
+    assert np.array_equal(rg[np.repeat(fl, d.ncomponents)], ro[np.repeat(fl, d.ncomponents)])
+    assert np.array_equal(ug[np.repeat(fl, d.ncomponents * d.ndims)], uo[np.repeat(fl, d.ncomponents * d.ndims)])
