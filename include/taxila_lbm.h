@@ -102,7 +102,7 @@ typedef struct txg_desc {
   int32_t nminerals;        /* -nminerals, 0..TXG_MAX_MINERALS */
   int32_t body_forces;      /* 1 if -gvt was given */
   int32_t bc_flags[6];      /* TXG_BC_* per face, XM..ZP */
-  int32_t reserved;         /* keeps the doubles 8-byte aligned; set 0 */
+  int32_t transport;        /* 0: the flow; 1: a passive-scalar species distribution (see below) */
   double gvt[3];            /* -gvt */
   double mm[TXG_MAX_COMPONENTS];    /* -mm_<name> */
   double tau[TXG_MAX_COMPONENTS];   /* -tau_<name> (SRT) */
@@ -120,6 +120,21 @@ typedef struct txg_desc {
   double bc_data[6][TXG_MAX_BCDATA];  /* option values of the face's BC (see TXG_BC_*) */
 } txg_desc;
 
+/* transport = 1 makes the handle a passive-scalar species distribution
+ * (LBM_Transport_module, lbm_transport.F90; -transport_discretization,
+ * -nspecies, -transport_relaxation_mode): ncomponents (1..4) species on
+ * the ndims lattice, each relaxed with its own tau / s_* (the species' relaxation,
+ * d_k = 0) towards the equilibrium of its own density and momentum
+ * (TransportCollision :381-431, TransportUpdateMoments :312-318), streamed
+ * and bounced back at walls (TransportReactWithWalls, non-reactive matrix);
+ * no forces and no face BCs (the reference does not apply transport BCs).
+ * On D2Q9 only nodes with walls > 0 collide (TransportCollisionD2 :450), so
+ * fluid populations stream unrelaxed, as the reference computes them.
+ * txg_set_state takes the species densities (psi) and momenta, txg_init is
+ * TransportFiInit (g = feq) + TransportUpdateMoments, txg_step the
+ * transport phases of LBMRun2, txg_get_fi / txg_get_moments return g, psi
+ * and the momenta (forces 0).  Flow sums, diagnostics, delta norm and
+ * batches are flow-only (TXG_ERR_UNSUPPORTED). */
 typedef struct txg_handle txg_handle;
 
 /* Fill d with the reference defaults: D3Q19, 1 component, SRT, tau = mm = 1,

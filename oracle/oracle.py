@@ -129,7 +129,8 @@ class Oracle:
         rt = np.empty(self.N)
         p = np.empty(self.N)
         v = np.empty(self.N * self.D)
-        lib().orc_diagnostics(self.h, _p(rt), _p(p), _p(v))
+        if lib().orc_diagnostics(self.h, _p(rt), _p(p), _p(v)):
+            raise RuntimeError("oracle: diagnostics are defined for the flow only")
         return rt, p, v
 
     def phase(self, *phases):
@@ -140,7 +141,8 @@ class Oracle:
 
     def delta_norm(self):
         out = np.zeros(1)
-        lib().orc_delta_norm(self.h, _p(out))
+        if lib().orc_delta_norm(self.h, _p(out)):
+            raise RuntimeError("oracle: the delta norm is defined for the flow only")
         return float(out[0])
 
 

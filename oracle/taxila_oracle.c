@@ -613,6 +613,15 @@ orc_t* orc_create(const txg_desc* d, char* err, int errlen) {
     for (int m = 0; m < o->C; m++)
       if (fabs(d->gw[k][m]) > eps15) o->fs = 1;
   o->body = d->body_forces != 0;
+  if (d->transport) {
+    /* a species distribution: no forces and no face BCs (LBMRun2 applies
+     * none to the transport, lbm.F90:290-370); RelaxationCreate's d_k = 0
+     * (lbm_relaxation.F90:81) */
+    if (o->ff || o->fs || o->body || d->nminerals) FAIL("species distributions take no forces");
+    for (int f = 0; f < 6; f++)
+      if (d->bc_flags[f]) FAIL("species distributions take no face boundary conditions");
+    for (int m = 0; m < o->C; m++) o->d_k[m] = 0.0;
+  }
   const char* bcerr = setup_bcs(o);
   if (bcerr) {
     if (err) snprintf(err, (size_t)errlen, "%s", bcerr);
@@ -1282,7 +1291,68 @@ static void bc_update_rho(orc_t* o) {
 /* ------------------------------------------------------------ driver */
 /* LBMInit2 flow part (lbm.F90:183-253): FlowFiInit (lbm_flow.F90:923-934)
  * unless fi came from a file, then FlowUpdateMoments (:466-478). */
+/* TransportCollisionD3/D2 (lbm_transport.F90:403-458) with
+ * RelaxationCollideSRT/MRT (lbm_relaxation.F90:171-200): no force
+ * prefactor; D3 collides fluid nodes, D2 the nodes with walls > 0 (as the
+ * reference's test reads) -- populations at wall nodes are never read
+ * back, so on D2 the fluid populations only stream. */
+static void collide_species(orc_t* o) {
+  if (o->D == 2) return;
+#pragma omp parallel for schedule(static)
+  for (int z = 1; z <= o->NZ; z++)
+    for (int y = 1; y <= o->NY; y++)
+      for (int x = 1; x <= o->NX; x++) {
+        if (WALL(o, x, y, z) != 0) continue;
+        for (int m = 0; m < o->C; m++) {
+          if (o->d.relaxation_mode == TXG_SRT) {
+            for (int n = 0; n <= o->b; n++)
+              FI(o, x, y, z, n, m) =
+                  FI(o, x, y, z, n, m) - (FI(o, x, y, z, n, m) - FEQ(o, x, y, z, n, m)) / o->tau[m];
+          } else {
+            double dfi[19];
+            for (int k = 0; k <= o->b; k++) dfi[k] = FI(o, x, y, z, k, m) - FEQ(o, x, y, z, k, m);
+            for (int n = 0; n <= o->b; n++) {
+              double mdiff = 0.0;
+              for (int k = 0; k <= o->b; k++) mdiff = mdiff + o->mt[n][k] * dfi[k];
+              for (int k = 0; k <= o->b; k++)
+                FI(o, x, y, z, k, m) =
+                    FI(o, x, y, z, k, m) - o->tau_mrt[m][n] * mdiff / o->mmt[n] * o->mt[n][k];
+            }
+          }
+        }
+      }
+}
+
+/* TransportFiInit (lbm_transport.F90:320-334): g = feq(psi, flux) where the
+ * species' flux is its momentum; then TransportUpdateMoments (:312-318) */
+static void init_species(orc_t* o) {
+  if (!o->fi_loaded) {
+    update_feq(o);
+    memcpy(o->fi, o->fi_eq, sizeof(double) * (size_t)(o->gN * o->Q * o->C));
+  }
+  calc_density(o);
+  calc_flux_into(o, o->flux);
+  o->initialized = 1;
+}
+
+/* the transport phases of LBMRun2 (lbm.F90:290-370) for a non-reactive
+ * matrix: TransportCollision, DistributionCommunicateFi, TransportStream,
+ * TransportReactWithWalls (bounce-back), TransportUpdateMoments */
+static void step_species(orc_t* o) {
+  update_feq(o);
+  collide_species(o);
+  halo(o, o->fi, o->Q * o->C, 0);
+  stream(o);
+  bounceback(o);
+  calc_density(o);
+  calc_flux_into(o, o->flux);
+}
+
 int orc_init(orc_t* o) {
+  if (o->d.transport) {
+    init_species(o);
+    return 0;
+  }
   if (!o->fi_loaded) {
     calc_forces(o); /* density halo inside */
     update_feq(o);
@@ -1300,6 +1370,10 @@ int orc_init(orc_t* o) {
 int orc_step(orc_t* o, int nsteps) {
   if (!o->initialized) return TXG_ERR_STATE;
   for (int s = 0; s < nsteps; s++) {
+    if (o->d.transport) {
+      step_species(o);
+      continue;
+    }
     update_feq(o);      /* FlowCollision */
     collide(o);
     halo(o, o->fi, o->Q * o->C, 0); /* DistributionCommunicateFi */
@@ -1347,6 +1421,7 @@ int orc_get_moments(const orc_t* o, double* rho, double* flux, double* forces) {
 
 /* FlowUpdateDiagnosticsD3/D2 (lbm_flow.F90:603-758), psi = rho */
 int orc_diagnostics(orc_t* o, double* rhot, double* prs, double* velt) {
+  if (o->d.transport) return TXG_ERR_UNSUPPORTED;
   long nf = o->gN * o->D * o->C;
   double* u = (double*)malloc(sizeof(double) * (size_t)nf);
   if (!u) return TXG_ERR_NOMEM;
@@ -1392,6 +1467,7 @@ int orc_diagnostics(orc_t* o, double* rhot, double* prs, double* velt) {
 /* DistributionCalcDeltaNorm on rho (lbm_distribution_function.F90:809-833);
  * rho_old starts at zero (VecSet, :173-174) */
 int orc_delta_norm(orc_t* o, double* norm) {
+  if (o->d.transport) return TXG_ERR_UNSUPPORTED;
   double mx = 0.0;
   long i = 0;
   for (int z = 1; z <= o->NZ; z++)

@@ -111,7 +111,7 @@ def build_fortran() -> str | None:
     if not os.path.exists(FLANG) or not os.path.isdir(FSRC):
         return None
     srcs = [os.path.join(FSRC, f) for f in ("txg_bindings.F90", "txg_options.F90", "txg_flow.F90",
-                                             "default_callbacks.F90", "taxila_gpu.F90")]
+                                             "txg_transport.F90", "default_callbacks.F90", "taxila_gpu.F90")]
     srcs = [s for s in srcs if os.path.exists(s)]
     if not srcs:
         return None

@@ -548,7 +548,10 @@ int run_export(txg_handle* h) {
   ExportPtrs e = export_ptrs(h);
   AnyBufs b = bufs(h, h->cur);
   int pulled = h->local_state ? 0 : 1;
-  rc = h->hip(h->ks.collide(h->params(), b.ptr(), pulled, pulled, &e, h->N, h->stream), "export kernel");
+  if (h->d.transport)
+    rc = h->hip(h->ks.species(h->params(), b.ptr(), pulled, &e, h->N, h->stream), "export kernel");
+  else
+    rc = h->hip(h->ks.collide(h->params(), b.ptr(), pulled, pulled, &e, h->N, h->stream), "export kernel");
   if (rc) return rc;
   return h->hip(hipStreamSynchronize(h->stream), "export sync");
 }
@@ -557,6 +560,7 @@ int run_export(txg_handle* h) {
 // nsums results land in dst (device memory), default h->sums' tail
 int queue_flow_sums(txg_handle* h, double* dst) {
   if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
+  if (h->d.transport) return h->fail(TXG_ERR_UNSUPPORTED, "flow-only call on a species distribution");
   // without a psi-gradient stencil k_sums forms each node's density itself
   int rc = h->d.isotropy_order != 0 ? ensure_rho(h) : TXG_OK;
   if (rc) return rc;
@@ -651,7 +655,6 @@ int txg_create(const txg_desc* d, txg_handle** out) {
   *out = nullptr;
   if (d->ndims != 2 && d->ndims != 3) return bad(TXG_ERR_ARG, "ndims must be 2 or 3");
   if (d->ncomponents < 1 || d->ncomponents > TXG_MAX_COMPONENTS) return bad(TXG_ERR_ARG, "ncomponents out of range");
-  if (d->ncomponents > 2) return bad(TXG_ERR_UNSUPPORTED, "the HIP step is built for 1 or 2 components");
   if (d->nx < 3 || d->ny < 3 || (d->ndims == 3 && d->nz < 3)) return bad(TXG_ERR_ARG, "domain must be at least 3 per side");
   const int iso = d->isotropy_order;
   if (!(iso == 0 || iso == 4 || iso == 8 || (iso == 10 && d->ndims == 2)))
@@ -666,6 +669,18 @@ int txg_create(const txg_desc* d, txg_handle** out) {
   for (int m = 0; m < d->ncomponents; m++) {
     if (!(d->mm[m] > 0)) return bad(TXG_ERR_ARG, "mm must be positive");
     if (d->relaxation_mode == TXG_SRT && !(d->tau[m] > 0.5)) return bad(TXG_ERR_ARG, "tau must exceed 1/2");
+  }
+  if (d->transport != 0 && d->transport != 1) return bad(TXG_ERR_ARG, "transport must be 0 (flow) or 1 (species)");
+  if (d->transport) {
+    // a species distribution: no forces and no face BCs (lbm_transport.F90)
+    for (int m = 0; m < d->ncomponents; m++)
+      for (int n = 0; n < d->ncomponents; n++)
+        if (d->g[m][n] != 0.0) return bad(TXG_ERR_ARG, "a species distribution takes no fluid-fluid forces");
+    if (d->body_forces || d->nminerals) return bad(TXG_ERR_ARG, "a species distribution takes no body or wall forces");
+    for (int f = 0; f < 6; f++)
+      if (d->bc_flags[f] != TXG_BC_NULL)
+        return bad(TXG_ERR_UNSUPPORTED, "species distributions have no face boundary conditions (the reference "
+                                        "applies none)");
   }
   BcSetup bcs;
   {
@@ -684,8 +699,12 @@ int txg_create(const txg_desc* d, txg_handle** out) {
   h->Np = (int)((N + 255) / 256 * 256);
   h->prec = d->precision;
   h->ssize = h->prec == TXG_FP64 ? 8 : (h->prec == TXG_FP32 ? 4 : 2);
-  bool ok = h->prec == TXG_FP64 ? kernels_fp64(h->D, h->C, iso, &h->ks)
-                                : kernels_fp32(h->D, h->C, iso, h->prec == TXG_FP32_FP16, &h->ks);
+  if (!d->transport && d->ncomponents > 2)
+    return bad(TXG_ERR_UNSUPPORTED, "the flow step is instantiated for 1 or 2 components (species "
+                                    "distributions: up to 4)");
+  const int kiso = d->transport ? 0 : iso;  // species: no psi-gradient stencil
+  bool ok = h->prec == TXG_FP64 ? kernels_fp64(h->D, h->C, kiso, &h->ks)
+                                : kernels_fp32(h->D, h->C, kiso, h->prec == TXG_FP32_FP16, &h->ks);
   if (!ok) {
     delete h;
     return bad(TXG_ERR_UNSUPPORTED, "no kernel instantiation for this lattice/components/isotropy");
@@ -702,8 +721,20 @@ int txg_create(const txg_desc* d, txg_handle** out) {
   if (d->body_forces) flags |= kFlagBody;
   if (d->relaxation_mode == TXG_MRT) flags |= kFlagMRT;
   if (iso == 0) flags &= ~kFlagFF;
+  if (d->transport) flags = (flags & kFlagMRT) | kFlagSpecies;
   fill_params<double>(*d, h->N, h->Np, flags, dirichlet, h->p64);
   fill_params<float>(*d, h->N, h->Np, flags, dirichlet, h->p32);
+  if (d->transport) {
+    // the species' relaxation keeps d_k = 0 (RelaxationCreate, lbm_relaxation.F90:77)
+    for (int m = 0; m < h->C; m++) {
+      h->p64.dk[m] = 0.0;
+      h->p64.a15[m] = 1.5;
+      h->p64.feq0a[m] = 1.0 / 6.0;
+      h->p32.dk[m] = 0.0f;
+      h->p32.a15[m] = 1.5f;
+      h->p32.feq0a[m] = (float)(1.0 / 6.0);
+    }
+  }
   if (h->prec == TXG_FP32_FP16)
     for (int m = 0; m < h->C; m++) h->p32.shift0[m] = (float)d->rho0[m];
   h->ks0 = h->ks;
@@ -1039,6 +1070,23 @@ int autotune(txg_handle* h, int reps) {
 int txg_step(txg_handle* h, int32_t nsteps) {
   if (!h || nsteps < 0) return TXG_ERR_ARG;
   if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
+  if (h->d.transport) {
+    // the transport phases of LBMRun2: collision, stream, wall bounce-back, moments
+    for (int s = 0; s < nsteps; s++) {
+      AnyBufs b = bufs(h, h->cur);
+      int rc;
+      {
+        ProfScope ps(h, 0);
+        rc = h->hip(h->ks.species(h->params(), b.ptr(), h->local_state ? 0 : 1, nullptr, h->N, h->stream),
+                    "species kernel");
+      }
+      if (rc) return rc;
+      h->cur = 1 - h->cur;
+      h->local_state = 0;
+      h->rho_valid = 0;
+    }
+    return TXG_OK;
+  }
   if (h->batch && nsteps > 0) batch_mark_dirty(h->batch);  // the batch's orientation snapshot goes stale
   const bool fused = use_fused(h);
   for (int s = 0; s < nsteps; s++) {
@@ -1086,6 +1134,8 @@ int txg_step(txg_handle* h, int32_t nsteps) {
 
 int txg_tune(txg_handle* h, int32_t kernel, int32_t tile_x, int32_t tile_y, int32_t zseg) {
   if (!h || kernel < 0 || kernel > 2 || zseg < 0) return TXG_ERR_ARG;
+  if (h->d.transport)
+    return h->fail(TXG_ERR_UNSUPPORTED, "species distributions step with their one kernel; nothing to tune");
   int tile = h->tile;
   if (tile_x != 0 || tile_y != 0) {
     tile = -1;
@@ -1182,17 +1232,23 @@ int txg_get_moments(txg_handle* h, double* rho, double* flux, double* forces) {
   if (rc) return rc;
   const size_t N = h->N, C = h->C, D = h->D;
   ExportPtrs e = export_ptrs(h);
-  std::vector<double> r(N * C), up(N * D), F(N * C * D);
+  std::vector<double> r(N * C), up(N * D), F(N * C * D), jm;
   rc = h->download(r.data(), e.rho, r.size() * 8, "rho download");
   if (!rc) rc = h->download(up.data(), e.up, up.size() * 8, "u download");
   if (!rc) rc = h->download(F.data(), e.F, F.size() * 8, "F download");
+  if (!rc && h->d.transport) {
+    // a species distribution's flux is its own momentum (DistributionCalcFlux)
+    jm.resize(N * C * D);
+    rc = h->download(jm.data(), e.j, jm.size() * 8, "j download");
+  }
   if (rc) return rc;
   for (size_t i = 0; i < N; i++) {
     const bool solid = h->hsolid[i];
     for (size_t m = 0; m < C; m++) {
       if (rho) rho[i * C + m] = solid ? 0.0 : r[m * N + i];
       for (size_t d = 0; d < D; d++) {
-        if (flux) flux[(i * D + d) * C + m] = solid ? 0.0 : up[d * N + i];
+        if (flux)
+          flux[(i * D + d) * C + m] = solid ? 0.0 : (h->d.transport ? jm[(m * D + d) * N + i] : up[d * N + i]);
         if (forces) forces[(i * D + d) * C + m] = solid ? 0.0 : F[(m * D + d) * N + i];
       }
     }
@@ -1202,6 +1258,7 @@ int txg_get_moments(txg_handle* h, double* rho, double* flux, double* forces) {
 
 int txg_diagnostics(txg_handle* h, double* rhot, double* prs, double* velt) {
   if (!h) return TXG_ERR_ARG;
+  if (h->d.transport) return h->fail(TXG_ERR_UNSUPPORTED, "flow-only call on a species distribution");
   if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
   // computed on the GPU from the populations in place (no state export);
   // without a psi-gradient stencil the kernel forms the densities itself
@@ -1227,6 +1284,7 @@ int txg_diagnostics(txg_handle* h, double* rhot, double* prs, double* velt) {
 
 int txg_reduce(txg_handle* h, double* out) {
   if (!h || !out) return TXG_ERR_ARG;
+  if (h->d.transport) return h->fail(TXG_ERR_UNSUPPORTED, "flow-only call on a species distribution");
   int rc = queue_flow_sums(h, nullptr);
   if (rc) return rc;
   rc = h->hip(hipStreamSynchronize(h->stream), "reduce sync");
@@ -1248,6 +1306,7 @@ int txg_flow_sums(txg_handle* h, double* out) {
 
 int txg_delta_norm(txg_handle* h, double* norm) {
   if (!h || !norm) return TXG_ERR_ARG;
+  if (h->d.transport) return h->fail(TXG_ERR_UNSUPPORTED, "flow-only call on a species distribution");
   if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
   // the densities come from the populations in place (no state export);
   // without a psi-gradient stencil the kernel forms them itself
@@ -1546,8 +1605,8 @@ int txg_batch_add(txg_batch* b, txg_handle* h) {
   if (!b || !h) return TXG_ERR_ARG;
   if (h->batch) return b->fail(TXG_ERR_STATE, "handle already belongs to a batch");
   if (!h->initialized) return b->fail(TXG_ERR_STATE, "txg_init has not been called on the handle");
-  if (h->D != 3 || !h->ks.fused_batch[first_tile(h->ks.fused_batch)])
-    return b->fail(TXG_ERR_UNSUPPORTED, "batched steps need a D3Q19 handle");
+  if (h->D != 3 || h->d.transport || !h->ks.fused_batch[first_tile(h->ks.fused_batch)])
+    return b->fail(TXG_ERR_UNSUPPORTED, "batched steps need a D3Q19 flow handle");
   if ((uint64_t)h->Q * h->Np * h->ssize >= (1ull << 32))
     return b->fail(TXG_ERR_UNSUPPORTED, "batched steps need < 4 GiB of populations per component");
   const int iso = h->d.isotropy_order;

@@ -47,6 +47,9 @@ struct KernelSet {
   // steady-state delta norm of the density (k_delta_state): block maxima to
   // part[kSumBlocks], old density [C*N] updated
   hipError_t (*delta)(const void* p, const void* b, int pulled, double* old, int first, double* part, hipStream_t s);
+  // one step of a passive-scalar species distribution (k_species); with
+  // ex != nullptr the state export instead (null unless isotropy 0)
+  hipError_t (*species)(const void* p, const void* b, int pulled, const ExportPtrs* ex, int N, hipStream_t s);
   // FlowUpdateDiagnostics (k_diag): rhot[N], prs[N], velt[N*D] in FP64;
   // dc: const DiagConst* (mm and g in FP64)
   hipError_t (*diag)(const void* p, const void* b, int pulled, const void* dc, double* rhot, double* prs,

@@ -46,6 +46,7 @@ constexpr int kFaceShift = 24;               // bits 24..29: node on face XM..ZP
 
 // flags
 constexpr int kFlagFF = 1, kFlagFS = 2, kFlagBody = 4, kFlagMRT = 8;
+constexpr int kFlagSpecies = 16;  // a passive-scalar species distribution (no forces, own-momentum equilibrium)
 // face BC types (the reference's bc%flags values, include/lbm_definitions.h:32-35)
 constexpr int kBcReflect = 2, kBcDirichlet = 3, kBcNeumann = 4, kBcVelocity = 5;
 
@@ -1043,20 +1044,38 @@ __device__ __forceinline__ void moments_forces(const Params<R>& p, const uint8_t
 // EXACT (FAST = false) keeps the reference's operations and their order;
 // FAST folds the divisions into reciprocals and the SRT update into
 // f' = (1-w) f + feq (w + pref (1 - w/2)).
-template <int D, int C, bool FAST, class R, class ST>
+// SPEC: a passive-scalar species distribution (TransportCollision,
+// lbm_transport.F90:381-431): each species' equilibrium takes its own
+// momentum um[m] (Σ g c, TransportUpdateMoments :312-318) as the velocity,
+// there is no forcing prefactor, and the relaxation is RelaxationCollide's
+// f - (f - feq)/τ (SRT) or its MRT form on f - feq.
+template <int D, int C, bool FAST, class R, class ST, bool SPEC = false>
 __device__ __forceinline__ void collide_store(const Params<R>& p, const ST& store, R (&f)[C][Lat<D>::Q],
-                                              const R (&rl)[C], const R (&F)[C][3], const R (&up)[3]) {
+                                              const R (&rl)[C], const R (&F)[C][3], const R (&up_common)[3],
+                                              const R (*um)[3] = nullptr) {
   using L = Lat<D>;
   constexpr int Q = L::Q;
-  R usqr = (R)0;
+  R usqr_common = (R)0;
+  if constexpr (!SPEC) {
 #pragma unroll
-  for (int d = 0; d < D; d++) usqr = usqr + up[d] * up[d];
+    for (int d = 0; d < D; d++) usqr_common = usqr_common + up_common[d] * up_common[d];
+  }
 #pragma unroll
   for (int m = 0; m < C; m++) {
+    const R* up = SPEC ? um[m] : up_common;
+    R usqr = usqr_common;
+    if constexpr (SPEC) {
+#pragma unroll
+      for (int d = 0; d < D; d++) usqr = usqr + up[d] * up[d];
+    }
     store.begin(m, f[m]);  // the component's populations before the collision
     const R r = rl[m];
     const R rcs = r * p.cs2[m];
     R feq[Q], pref[Q];
+    if constexpr (SPEC) {
+#pragma unroll
+      for (int n = 0; n < Q; n++) pref[n] = (R)0;
+    }
     if constexpr (FAST) {
       const R irc = recip<FAST>(rcs);
       R Fu = (R)0;
@@ -1078,7 +1097,7 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, const ST& stor
             ue = ue - up[d];
           }
         });
-        pref[n] = (Fc - Fu) * irc;
+        if constexpr (!SPEC) pref[n] = (Fc - Fu) * irc;
         if constexpr (n == 0) {
           if constexpr (D == 3)
             feq[0] = r * (p.dk[m] - usqr * (R)0.5);
@@ -1092,12 +1111,14 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, const ST& stor
       sfor<0, Q>([&](auto NI) {
         constexpr int n = decltype(NI)::value;
         // prefactor: sum_d F_d (c_nd - u_d) / (rho c_s^2)
-        R s = (R)0;
-        sfor<0, D>([&](auto DI) {
-          constexpr int d = decltype(DI)::value;
-          s = s + F[m][d] * ((R)cdir<D>(n, d) - up[d]);
-        });
-        pref[n] = s / rcs;
+        if constexpr (!SPEC) {
+          R s = (R)0;
+          sfor<0, D>([&](auto DI) {
+            constexpr int d = decltype(DI)::value;
+            s = s + F[m][d] * ((R)cdir<D>(n, d) - up[d]);
+          });
+          pref[n] = s / rcs;
+        }
         if constexpr (n == 0) {
           if constexpr (D == 3)
             feq[0] = r * (p.dk[m] - usqr / (R)2.);
@@ -1117,7 +1138,10 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, const ST& stor
       });
     }
     if (!(p.flags & kFlagMRT)) {
-      if constexpr (FAST) {
+      if constexpr (SPEC && !FAST) {
+#pragma unroll
+        for (int n = 0; n < Q; n++) f[m][n] = f[m][n] - (f[m][n] - feq[n]) / p.tau[m];
+      } else if constexpr (FAST) {
         const R w = p.omega[m], w1 = (R)1 - w, wh = (R)1 - (R)0.5 * w;
 #pragma unroll
         for (int n = 0; n < Q; n++) f[m][n] = w1 * f[m][n] + feq[n] * (w + pref[n] * wh);
@@ -1132,8 +1156,13 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, const ST& stor
       }
     } else {
       R dfi[Q];
+      if constexpr (SPEC) {
 #pragma unroll
-      for (int n = 0; n < Q; n++) dfi[n] = f[m][n] - ((R)1. - (R)0.5 * pref[n]) * feq[n];
+        for (int n = 0; n < Q; n++) dfi[n] = f[m][n] - feq[n];
+      } else {
+#pragma unroll
+        for (int n = 0; n < Q; n++) dfi[n] = f[m][n] - ((R)1. - (R)0.5 * pref[n]) * feq[n];
+      }
       sfor<0, Q>([&](auto KI) {
         constexpr int k = decltype(KI)::value;
         R md = (R)0;
@@ -1157,8 +1186,10 @@ __device__ __forceinline__ void collide_store(const Params<R>& p, const ST& stor
           else if constexpr (mv != 0) f[m][n] = f[m][n] - t * (R)mv;
         });
       });
+      if constexpr (!SPEC) {
 #pragma unroll
-      for (int n = 0; n < Q; n++) f[m][n] = f[m][n] + pref[n] * feq[n];
+        for (int n = 0; n < Q; n++) f[m][n] = f[m][n] + pref[n] * feq[n];
+      }
     }
     // the rest population last: a mass-conserving store forms it from the
     // others as stored
@@ -1281,6 +1312,107 @@ __global__ void __launch_bounds__(256) k_collide(Params<R> p, Bufs<S, R> b, int 
   R res[MAXC];
   pad_res<C, R>(d0, res);
   collide_store<D, C, FAST, R>(p, PtrStore<D, S, R>(p, b, node, res), f, rl, F, up);
+}
+
+// One step of a passive-scalar species distribution (the transport phases
+// of LBMRun2, lbm.F90:279-406: TransportCollision, DistributionStream,
+// TransportReactWithWalls -> DistributionBounceback, TransportUpdateMoments),
+// pulled: the populations are pulled through the bounce-back link masks
+// (no boundary conditions: TransportApplyBCs is not called), the moments
+// ρ_m = Σ g and j_m = Σ g c are formed, and the next collision relaxes each
+// species towards feq(ρ_m, j_m) (collide_store, SPEC).  In 2-D the reference
+// collides wall nodes only (TransportCollisionD2 tests walls > 0,
+// lbm_transport.F90:449), so fluid nodes keep their streamed populations;
+// that is reproduced.  pulled = 0 reads the state fresh from txg_init;
+// ex.f != nullptr exports (g, ρ, j) instead of stepping.
+template <int D, int C, class S, class R>
+__global__ void __launch_bounds__(256) k_species(Params<R> p, Bufs<S, R> b, int pulled, Export<D, C, R> ex) {
+  using L = Lat<D>;
+  constexpr int Q = L::Q;
+  constexpr bool FAST = !std::is_same<R, double>::value;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= p.N) return;
+  const uint32_t mk = b.mask[node];
+  if (mk & kSelfSolid) {
+    if (ex.f) {
+#pragma unroll
+      for (int m = 0; m < C; m++) {
+#pragma unroll
+        for (int n = 0; n < Q; n++) ex.f[(size_t)(m * Q + n) * p.N + node] = 0.0;
+        ex.rho[(size_t)m * p.N + node] = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          ex.F[(size_t)(m * D + d) * p.N + node] = 0.0;
+          ex.j[(size_t)(m * D + d) * p.N + node] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < D; d++) ex.up[(size_t)d * p.N + node] = 0.0;
+    }
+    return;
+  }
+  R f[C][Q], d0[C];
+  if (pulled) {
+    int x, y, z;
+    node_xyz(p, node, x, y, z);
+    Nbr nb;
+    make_nbr<D>(p, x, y, z, nb);
+    pull<D, C, S, R>(p, b, node, mk, nb, f, d0);
+  } else {
+    load_local<D, C, S, R>(p, b, node, f, d0);
+  }
+  R rl[C], j[C][3], F[C][3], u0[3] = {(R)0, (R)0, (R)0};
+#pragma unroll
+  for (int m = 0; m < C; m++) {
+    R sr = (R)0;
+#pragma unroll
+    for (int n = 0; n < Q; n++) sr = sr + f[m][n];
+    rl[m] = sr;
+    j[m][2] = (R)0;
+    sfor<0, D>([&](auto DI) {
+      constexpr int d = decltype(DI)::value;
+      R a = (R)0;
+      sfor<0, Q>([&](auto NI) {
+        constexpr int n = decltype(NI)::value;
+        constexpr int c = cdir<D>(n, d);
+        if constexpr (c == 1) a = a + f[m][n];
+        if constexpr (c == -1) a = a - f[m][n];
+      });
+      j[m][d] = a;
+    });
+    F[m][0] = F[m][1] = F[m][2] = (R)0;
+  }
+  if (ex.f) {
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+#pragma unroll
+      for (int n = 0; n < Q; n++) ex.f[(size_t)(m * Q + n) * p.N + node] = (double)f[m][n];
+      ex.f[(size_t)(m * Q) * p.N + node] = (double)f[m][0] + (double)d0[m];
+      ex.rho[(size_t)m * p.N + node] = (double)rl[m];
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        ex.F[(size_t)(m * D + d) * p.N + node] = 0.0;
+        ex.j[(size_t)(m * D + d) * p.N + node] = (double)j[m][d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) ex.up[(size_t)d * p.N + node] = 0.0;
+    return;
+  }
+  R res[MAXC];
+  pad_res<C, R>(d0, res);
+  const PtrStore<D, S, R> store(p, b, node, res);
+  if constexpr (D == 3) {
+    collide_store<D, C, FAST, R, PtrStore<D, S, R>, true>(p, store, f, rl, F, u0, j);
+  } else {
+#pragma unroll
+    for (int m = 0; m < C; m++) {
+      store.begin(m, f[m]);
+#pragma unroll
+      for (int n = 1; n < Q; n++) store(m, n, f[m][n]);
+      store(m, 0, f[m][0]);
+    }
+  }
 }
 
 // ------------------------------------------------------------ flow sums
@@ -1530,7 +1662,9 @@ __global__ void __launch_bounds__(256) k_fiinit(Params<R> p, Bufs<S, R> b, const
         constexpr int d = decltype(DI)::value;
         s = s + F[m][d] * ((R)cdir<D>(n, d) - u[d]);
       });
-      const R pref = s / (r * p.cs2[m]);
+      // a species distribution starts at feq itself (TransportFiInit_,
+      // lbm_transport.F90:328-334)
+      const R pref = (p.flags & kFlagSpecies) ? (R)0 : s / (r * p.cs2[m]);
       R feq;
       if constexpr (n == 0) {
         if constexpr (D == 3)

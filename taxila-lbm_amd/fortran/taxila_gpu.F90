@@ -9,16 +9,22 @@
 ! initialize_walls callback) -> initial state (initialize_state callback, or
 ! -restart -istep k reading <prefix>fi<k/kwrite>.dat) -> init -> npasses
 ! steps with output every -kwrite steps and the optional steady-state exit.
+! With -transport_discretization the passive-scalar species distributions
+! step beside the flow (txg_transport.F90); their initial state comes from
+! the initialize_state_transport callback (zero on restart, as the
+! reference's restart reloads the flow only, lbm.F90:519-544).
 !
 ! Callbacks are linked at build time, as in the reference's test Makefiles:
 !   subroutine initialize_walls(walls, nx, ny, nz, opts)
 !   subroutine initialize_state(rho, u, walls, nx, ny, nz, nc, nd, opts)
+!   subroutine initialize_state_transport(psi, flux, walls, nx, ny, nz, ns, nd, opts)
 ! with plain owned arrays in the C ABI layouts (walls(node), rho(C*node),
 ! u(C*D*node), node = x + nx*(y + ny*z)).
 program taxila_gpu
   use iso_fortran_env, only: real64, int64
   use txg_options
   use txg_flow
+  use txg_transport
   implicit none
 
   interface
@@ -35,10 +41,18 @@ program taxila_gpu
        double precision, intent(in) :: walls(nx * ny * nz)
        type(options_type), intent(in) :: opts
      end subroutine initialize_state
+     subroutine initialize_state_transport(psi, flux, walls, nx, ny, nz, ns, nd, opts)
+       use txg_options
+       integer, intent(in) :: nx, ny, nz, ns, nd
+       double precision, intent(inout) :: psi(ns * nx * ny * nz), flux(ns * nd * nx * ny * nz)
+       double precision, intent(in) :: walls(nx * ny * nz)
+       type(options_type), intent(in) :: opts
+     end subroutine initialize_state_transport
   end interface
 
   type(options_type) :: opts
   type(flow_type) :: flow
+  type(transport_type) :: tr
   character(len=512) :: infile, prefix, wfile
   integer :: ierr, npasses, kwrite, istep, walls_type, counter, lcv, n, restart_counter
   logical :: flag, restart, run_ss, supress_ic, output_last
@@ -82,7 +96,9 @@ program taxila_gpu
   call OptionsGetString(opts, '-output_file_prefix', prefix, flag)
 
   call FlowSetFromOptions(flow, opts)
+  call TransportSetFromOptions(tr, opts, flow%desc)
   call FlowSetUp(flow)
+  if (tr%enabled) call TransportSetUp(tr)
 
   ! walls (lbm_walls.F90:151-177)
   allocate(walls(flow%nnodes))
@@ -105,6 +121,7 @@ program taxila_gpu
      call initialize_walls(walls, flow%nx, flow%ny, flow%nz, opts)
   end if
   call WallsSetValues(flow, walls)
+  if (tr%enabled) call TransportSetWalls(tr, walls)
 
   ! initial state (lbm.F90:444-453, restart lbm.F90:519-544)
   counter = 0
@@ -127,10 +144,23 @@ program taxila_gpu
      call initialize_state(rho, u, walls, flow%nx, flow%ny, flow%nz, flow%ncomponents, flow%ndims, opts)
      call FlowInitializeState(flow, rho, u)
   end if
+  if (tr%enabled) then
+     block
+       double precision, allocatable :: psi(:), flux(:)
+       allocate(psi(tr%nnodes * tr%nspecies), flux(tr%nnodes * tr%nspecies * tr%ndims))
+       psi = 0.d0
+       flux = 0.d0
+       if (.not. restart) &
+            call initialize_state_transport(psi, flux, walls, tr%nx, tr%ny, tr%nz, tr%nspecies, tr%ndims, opts)
+       call TransportInitializeState(tr, psi, flux)
+     end block
+  end if
   call FlowInit(flow)
+  if (tr%enabled) call TransportFiInit(tr)
   if (.not. restart .and. .not. supress_ic) then
      write(*, '(A,I0,A,I0)') ' outputing step ', istep, ' to file ', counter
      call FlowOutput(flow, prefix, counter)
+     if (tr%enabled) call TransportOutputDiagnostics(tr, prefix, counter)
   end if
   counter = counter + 1
 
@@ -138,14 +168,19 @@ program taxila_gpu
   lcv = istep
   do lcv = istep + 1, npasses
      call FlowCollision(flow)
+     if (tr%enabled) call TransportCollision(tr)
      call FlowStream(flow)
+     if (tr%enabled) call TransportStream(tr)
      call FlowBounceback(flow)
+     if (tr%enabled) call TransportReactWithWalls(tr)
      call FlowApplyBCs(flow)
      call FlowUpdateFlux(flow)
+     if (tr%enabled) call TransportUpdateMoments(tr)
      if (kwrite > 0) then
         if (mod(lcv, kwrite) == 0) then
            write(*, '(A,I0,A,I0)') ' outputing step ', lcv, ' to file ', counter
            call FlowOutput(flow, prefix, counter)
+           if (tr%enabled) call TransportOutputDiagnostics(tr, prefix, counter)
            counter = counter + 1
         end if
      end if
@@ -157,9 +192,13 @@ program taxila_gpu
         end if
      end if
   end do
-  if (output_last) call FlowOutput(flow, prefix, counter)
+  if (output_last) then
+     call FlowOutput(flow, prefix, counter)
+     if (tr%enabled) call TransportOutputDiagnostics(tr, prefix, counter)
+  end if
   call system_clock(t1)
   if (npasses > istep) write(*, '(A,ES12.4,A)') ' Timing: Simulation took ', &
        dble(t1 - t0) / dble(rate) / dble(max(1, npasses - istep)), ' s (per unit timestep)'
+  call TransportDestroy(tr)
   call FlowDestroy(flow)
 end program taxila_gpu

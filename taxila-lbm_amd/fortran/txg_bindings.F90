@@ -26,7 +26,7 @@ module txg_bindings
      integer(c_int32_t) :: periodic(3)
      integer(c_int32_t) :: isotropy_order, relaxation_mode, precision, nminerals, body_forces
      integer(c_int32_t) :: bc_flags(6)
-     integer(c_int32_t) :: reserved
+     integer(c_int32_t) :: transport
      real(c_double) :: gvt(3)
      real(c_double) :: mm(TXG_MAXC), tau(TXG_MAXC)
      real(c_double) :: s_c(TXG_MAXC), s_e(TXG_MAXC), s_e2(TXG_MAXC), s_q(TXG_MAXC)

@@ -1,0 +1,209 @@
+! txg_transport.F90 -- the passive-scalar species distributions of the
+! Fortran driver (LBM_Transport_module, lbm_transport.F90), backed by the
+! MI355X step library through ISO_C_BINDING as a second handle with
+! desc%transport = 1.
+!
+! Options are read under the reference's names (lbm_options.F90:326-360,
+! lbm_species.F90:111-133, lbm_relaxation.F90:117-151):
+!   -transport_discretization d3q19|d2q9 (must match the flow's),
+!   -nspecies, -transport_relaxation_mode 0|1, -specie<i>_name,
+!   -tau_<name> or -s_c_<name> ... -s_m_<name>, -bc_periodic_{x,y,z}.
+! -reactive_matrix (dissolution/precipitation) is refused.  The reference
+! parses -bc_conc_* but LBMRun2 never applies transport BCs (lbm.F90:348-350
+! are commented out), so they are not read here.  The procedure names keep
+! LBMRun2's transport call sites: TransportCollision queues the whole
+! transport step; TransportStream, TransportReactWithWalls and
+! TransportUpdateMoments remain as no-op call sites.
+module txg_transport
+  use iso_c_binding
+  use iso_fortran_env, only: real64
+  use txg_bindings
+  use txg_options
+  use txg_flow, only: write_petsc_vec
+  implicit none
+  private
+
+  type, public :: transport_type
+     type(txg_desc) :: desc
+     type(c_ptr) :: h = c_null_ptr
+     logical :: enabled = .false.
+     integer :: ndims = 3, nspecies = 1, nx = 0, ny = 0, nz = 1, nnodes = 0, q = 19
+     character(len=64) :: names(TXG_MAXC)
+  end type transport_type
+
+  public :: TransportSetFromOptions, TransportSetUp, TransportSetWalls, TransportInitializeState, &
+       TransportLoadFi, TransportFiInit, TransportCollision, TransportStream, TransportReactWithWalls, &
+       TransportUpdateMoments, TransportOutputDiagnostics, TransportDestroy
+
+contains
+
+  subroutine TransportCheck(tr, rc, what)
+    type(transport_type), intent(in) :: tr
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= TXG_OK) then
+       write(*, '(A,A,A,I0,A,A)') 'ERROR in ', what, ' (', rc, '): ', trim(txg_error_text(tr%h))
+       error stop 1
+    end if
+  end subroutine TransportCheck
+
+  character(len=8) function itoa(i)
+    integer, intent(in) :: i
+    write(itoa, '(I0)') i
+  end function itoa
+
+  ! TransportSetFromOptions (lbm_transport.F90:130-245) on the flow's grid
+  subroutine TransportSetFromOptions(tr, opts, flow_desc)
+    type(transport_type), intent(inout) :: tr
+    type(options_type), intent(in) :: opts
+    type(txg_desc), intent(in) :: flow_desc
+    character(len=64) :: disc
+    logical :: flag, reactive
+    integer :: m, mode
+    integer(c_int) :: rc
+
+    disc = ''
+    call OptionsGetString(opts, '-transport_discretization', disc, flag)
+    tr%enabled = flag
+    if (.not. flag) return
+    select case (lower(trim(disc)))
+    case ('d3q19')
+       tr%ndims = 3
+       tr%q = 19
+    case ('d2q9')
+       tr%ndims = 2
+       tr%q = 9
+    case default
+       write(*, *) 'ERROR: Invalid Discretization ', trim(disc)
+       error stop 1
+    end select
+    if (tr%ndims /= flow_desc%ndims) then
+       write(*, *) "ERROR: Flow and transport discretization dimensions don't match"
+       error stop 1
+    end if
+    reactive = .false.
+    call OptionsGetBool(opts, '-reactive_matrix', reactive, flag)
+    if (reactive) then
+       write(*, *) 'ERROR: -reactive_matrix (dissolution/precipitation) is not supported'
+       error stop 1
+    end if
+    rc = txg_desc_init(tr%desc)
+    tr%desc%transport = 1
+    tr%desc%ndims = tr%ndims
+    tr%desc%nx = flow_desc%nx
+    tr%desc%ny = flow_desc%ny
+    tr%desc%nz = flow_desc%nz
+    tr%desc%periodic = flow_desc%periodic
+    tr%desc%precision = flow_desc%precision
+    tr%desc%isotropy_order = 0
+    tr%nx = flow_desc%nx
+    tr%ny = flow_desc%ny
+    tr%nz = flow_desc%nz
+    tr%nnodes = tr%nx * tr%ny * tr%nz
+    tr%nspecies = 1
+    call OptionsGetInt(opts, '-nspecies', tr%nspecies, flag)
+    if (tr%nspecies < 1 .or. tr%nspecies > TXG_MAXC) then
+       write(*, *) 'ERROR: -nspecies must be 1 ..', TXG_MAXC
+       error stop 1
+    end if
+    tr%desc%ncomponents = tr%nspecies
+    mode = 0
+    call OptionsGetInt(opts, '-transport_relaxation_mode', mode, flag)
+    tr%desc%relaxation_mode = mode
+    ! species (SpecieCreateN names specie<i>, lbm_species.F90:60-79)
+    do m = 1, tr%nspecies
+       tr%names(m) = 'specie' // trim(itoa(m))
+       call OptionsGetString(opts, '-' // trim(tr%names(m)) // '_name', tr%names(m), flag)
+       if (mode == TXG_SRT) then
+          call OptionsGetReal(opts, '-tau_' // trim(tr%names(m)), tr%desc%tau(m), flag)
+       else
+          call OptionsGetReal(opts, '-s_c_' // trim(tr%names(m)), tr%desc%s_c(m), flag)
+          call OptionsGetReal(opts, '-s_e_' // trim(tr%names(m)), tr%desc%s_e(m), flag)
+          call OptionsGetReal(opts, '-s_e2_' // trim(tr%names(m)), tr%desc%s_e2(m), flag)
+          call OptionsGetReal(opts, '-s_q_' // trim(tr%names(m)), tr%desc%s_q(m), flag)
+          call OptionsGetReal(opts, '-s_nu_' // trim(tr%names(m)), tr%desc%s_nu(m), flag)
+          call OptionsGetReal(opts, '-s_pi_' // trim(tr%names(m)), tr%desc%s_pi(m), flag)
+          call OptionsGetReal(opts, '-s_m_' // trim(tr%names(m)), tr%desc%s_m(m), flag)
+       end if
+    end do
+  end subroutine TransportSetFromOptions
+
+  subroutine TransportSetUp(tr)
+    type(transport_type), intent(inout) :: tr
+    call TransportCheck(tr, txg_create(tr%desc, tr%h), 'txg_create (transport)')
+  end subroutine TransportSetUp
+
+  ! the flow's walls (the transport shares lbm%walls)
+  subroutine TransportSetWalls(tr, walls)
+    type(transport_type), intent(inout) :: tr
+    real(real64), intent(in) :: walls(:)
+    call TransportCheck(tr, txg_set_walls(tr%h, walls), 'txg_set_walls (transport)')
+  end subroutine TransportSetWalls
+
+  ! what the initialize_state_transport callback produced: psi(S,node) and
+  ! the species fluxes flux(S,D,node) (LBMInitializeState_FlowTransport,
+  ! lbm.F90:455-472)
+  subroutine TransportInitializeState(tr, psi, flux)
+    type(transport_type), intent(inout) :: tr
+    real(real64), intent(in) :: psi(:), flux(:)
+    call TransportCheck(tr, txg_set_state(tr%h, psi, flux), 'txg_set_state (transport)')
+  end subroutine TransportInitializeState
+
+  subroutine TransportLoadFi(tr, gi)
+    type(transport_type), intent(inout) :: tr
+    real(real64), intent(in) :: gi(:)
+    call TransportCheck(tr, txg_set_fi(tr%h, gi), 'txg_set_fi (transport)')
+  end subroutine TransportLoadFi
+
+  ! TransportFiInit + TransportUpdateMoments (lbm.F90:215-245)
+  subroutine TransportFiInit(tr)
+    type(transport_type), intent(inout) :: tr
+    call TransportCheck(tr, txg_init(tr%h), 'txg_init (transport)')
+  end subroutine TransportFiInit
+
+  ! one whole transport step, queued on the GPU (TransportCollision through
+  ! TransportUpdateMoments)
+  subroutine TransportCollision(tr)
+    type(transport_type), intent(inout) :: tr
+    call TransportCheck(tr, txg_step(tr%h, 1_c_int32_t), 'txg_step (transport)')
+  end subroutine TransportCollision
+
+  subroutine TransportStream(tr)
+    type(transport_type), intent(inout) :: tr
+  end subroutine TransportStream
+
+  subroutine TransportReactWithWalls(tr)
+    type(transport_type), intent(inout) :: tr
+  end subroutine TransportReactWithWalls
+
+  subroutine TransportUpdateMoments(tr)
+    type(transport_type), intent(inout) :: tr
+  end subroutine TransportUpdateMoments
+
+  ! TransportOutputDiagnostics (lbm_transport.F90:342-356): io_fi and io_rho
+  ! are always on there, so gi<NNN>.dat and psi<NNN>.dat
+  subroutine TransportOutputDiagnostics(tr, prefix, counter)
+    type(transport_type), intent(inout) :: tr
+    character(len=*), intent(in) :: prefix
+    integer, intent(in) :: counter
+    real(real64), allocatable :: a(:), b(:), c(:)
+    character(len=3) :: cnt
+    write(cnt, '(I3.3)') counter
+    allocate(a(tr%nnodes * tr%q * tr%nspecies))
+    call TransportCheck(tr, txg_get_fi(tr%h, a), 'txg_get_fi (transport)')
+    call write_petsc_vec(trim(prefix) // 'gi' // cnt // '.dat', a)
+    deallocate(a)
+    allocate(a(tr%nnodes * tr%nspecies), b(tr%nnodes * tr%nspecies * tr%ndims), &
+         c(tr%nnodes * tr%nspecies * tr%ndims))
+    call TransportCheck(tr, txg_get_moments(tr%h, a, b, c), 'txg_get_moments (transport)')
+    call write_petsc_vec(trim(prefix) // 'psi' // cnt // '.dat', a)
+    deallocate(a, b, c)
+  end subroutine TransportOutputDiagnostics
+
+  subroutine TransportDestroy(tr)
+    type(transport_type), intent(inout) :: tr
+    integer(c_int) :: rc
+    if (c_associated(tr%h)) rc = txg_destroy(tr%h)
+    tr%h = c_null_ptr
+  end subroutine TransportDestroy
+end module txg_transport

@@ -51,7 +51,7 @@ class Desc(C.Structure):
         ("nminerals", C.c_int32),
         ("body_forces", C.c_int32),
         ("bc_flags", C.c_int32 * 6),
-        ("reserved", C.c_int32),
+        ("transport", C.c_int32),
         ("gvt", C.c_double * 3),
         ("mm", C.c_double * MAXC),
         ("tau", C.c_double * MAXC),
@@ -82,7 +82,7 @@ def make_desc(ndims=3, ncomponents=1, shape=(16, 16, 16), periodic=(1, 1, 1),
               isotropy_order=4, relaxation_mode=SRT, precision=FP64,
               mm=None, tau=None, s_c=None, s_e=None, s_e2=None, s_q=None,
               s_nu=None, s_pi=None, s_m=None, g=None, gw=None, gvt=None,
-              bc=None, rho0=None) -> Desc:
+              bc=None, rho0=None, transport=0) -> Desc:
     """Build a descriptor with the reference defaults (lbm_options.F90:93-152,
     lbm_relaxation.F90:58-80) overridden by the given values.
 
@@ -90,8 +90,10 @@ def make_desc(ndims=3, ncomponents=1, shape=(16, 16, 16), periodic=(1, 1, 1),
     -gw values; bc a dict face -> either a list of per-component densities
     (-bc_density_<face>) or a pair (BC_* code, option values) with the
     values of the reference's flow%bc_data(:, face) (see TXG_BC_* in
-    include/taxila_lbm.h)."""
+    include/taxila_lbm.h).  transport=1 makes a passive-scalar species
+    distribution (ncomponents species) instead of the flow."""
     d = Desc()
+    d.transport = int(transport)
     d.ndims = ndims
     d.ncomponents = ncomponents
     d.nx, d.ny = shape[0], shape[1]

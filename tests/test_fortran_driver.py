@@ -38,6 +38,46 @@ def test_missing_options_file(tmp_path):
 
 
 @needs_driver
+def test_transport_options_are_checked(tmp_path):
+    r = subprocess.run([EXE, OPTS, "-transport_discretization", "d3q19"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode != 0
+    assert "dimensions don't match" in r.stdout + r.stderr
+    r = subprocess.run([EXE, OPTS, "-transport_discretization", "d2q9", "-reactive_matrix"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "reactive_matrix" in r.stdout + r.stderr
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_species_through_fortran_driver(tmp_path):
+    """bubble_2D with two passive-scalar species beside the flow: the gi and
+    psi outputs at step 100 equal the FP64 oracle's species run."""
+    import oracle
+
+    out = tmp_path / "test_solution"
+    out.mkdir()
+    args = ["-transport_discretization", "d2q9", "-nspecies", "2", "-specie1_name", "tracer",
+            "-tau_tracer", "0.8", "-tau_specie2", "1.3", "-psi_inner", "1.0,0.2", "-psi_outer", "0.1,0.0"]
+    r = subprocess.run([EXE, OPTS, "-output_file_prefix", str(out) + "/"] + args, cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = txg.make_desc(ndims=2, ncomponents=2, shape=(128, 128), periodic=(1, 1), isotropy_order=0,
+                      tau=[0.8, 1.3], transport=1)
+    psi0 = cases.square_ic((128, 128), 26, (1.0, 0.2), (0.1, 0.0))
+    o = oracle.Oracle(d)
+    o.set_walls(np.zeros(d.nnodes))
+    o.set_state(psi0, np.zeros(d.nnodes * 4))
+    o.init()
+    g0 = txg.read_petsc_vec(str(out / "gi000.dat"))
+    assert np.array_equal(g0, o.get_fi())
+    o.step(100)
+    assert np.array_equal(txg.read_petsc_vec(str(out / "gi001.dat")), o.get_fi())
+    assert np.array_equal(txg.read_petsc_vec(str(out / "psi001.dat")), o.get_moments()[0])
+
+
+@needs_driver
 @pytest.mark.gpu
 def test_bubble_2d_through_fortran_driver(tmp_path):
     out = tmp_path / "test_solution"

@@ -1,0 +1,130 @@
+"""Passive-scalar species distributions on the HIP path vs the FP64 oracle (GPU).
+
+The transport phases of LBMRun2 (lbm.F90:290-370; TransportCollision,
+TransportStream, TransportReactWithWalls with a non-reactive matrix,
+TransportUpdateMoments in lbm_transport.F90) run as one kernel per step,
+k_species.  FP64 must match the oracle bit for bit at fluid nodes -- the
+populations g, the species densities psi and their fluxes -- on D3Q19 and
+D2Q9, SRT and MRT, one to four species, with walls and closed or periodic
+faces, through a
+restart from populations and through exports between steps.  FP32 is held
+to a relative tolerance and to mass conservation.
+"""
+import numpy as np
+import pytest
+
+import cases
+import oracle
+import txg
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (name, desc kwargs, wall fraction, steps)
+    ("d3_srt_c2_porous", dict(ndims=3, ncomponents=2, shape=(24, 20, 18), periodic=(1, 0, 1), tau=[0.8, 1.3]),
+     0.15, 25),
+    ("d3_mrt_c1_closed", dict(ndims=3, ncomponents=1, shape=(20, 18, 22), periodic=(0, 0, 0),
+                              relaxation_mode=txg.MRT, s_c=[1.0], s_e=[1.19], s_e2=[1.4], s_q=[1.2],
+                              s_nu=[1 / 0.7], s_pi=[1.1], s_m=[0.98]), 0.1, 25),
+    ("d3_mrt_c3_periodic", dict(ndims=3, ncomponents=3, shape=(16, 16, 16), relaxation_mode=txg.MRT,
+                                s_c=[1.0] * 3, s_e=[1.1, 1.2, 1.3], s_e2=[1.4] * 3, s_q=[1.2] * 3,
+                                s_nu=[1.5, 1.0, 0.9], s_pi=[1.0] * 3, s_m=[1.0] * 3), 0.0, 20),
+    ("d2_srt_c3_porous", dict(ndims=2, ncomponents=3, shape=(48, 36), periodic=(0, 1), tau=[0.7, 1.0, 1.6]),
+     0.15, 40),
+    ("d3_srt_c4_porous", dict(ndims=3, ncomponents=4, shape=(14, 12, 16), periodic=(1, 1, 0),
+                              tau=[0.6, 0.9, 1.0, 1.9]), 0.2, 15),
+    ("d2_mrt_c1_closed", dict(ndims=2, ncomponents=1, shape=(40, 30), periodic=(0, 0), relaxation_mode=txg.MRT,
+                              s_c=[1.0], s_e=[1.1], s_e2=[1.2], s_q=[1.3], s_nu=[1.4]), 0.1, 40),
+]
+
+
+def build(kw, frac, precision=txg.FP64):
+    d = txg.make_desc(transport=1, isotropy_order=0, precision=precision, **kw)
+    shape = (d.nx, d.ny, d.nz) if d.ndims == 3 else (d.nx, d.ny)
+    walls = cases.random_walls(shape, frac, seed=21) if frac else np.zeros(d.nnodes)
+    rng = np.random.default_rng(7)
+    fl = np.repeat(walls == 0, d.ncomponents)
+    rho = (0.05 + rng.random(d.nnodes * d.ncomponents)) * fl
+    u = 0.01 * rng.standard_normal(d.nnodes * d.ndims * d.ncomponents)
+    return d, walls, rho, u
+
+
+def run(cls, d, walls, rho, u, steps, fi=None):
+    dom = cls(d)
+    dom.set_walls(walls)
+    dom.set_state(rho, u)
+    if fi is not None:
+        dom.set_fi(fi)
+    dom.init()
+    dom.step(steps)
+    return dom
+
+
+def assert_same(g, o, d, walls):
+    fl = walls == 0
+    C, D, Q = d.ncomponents, d.ndims, d.Q
+    fg, fo = g.get_fi(), o.get_fi()
+    m = np.repeat(fl, Q * C)
+    assert np.isfinite(fo[m]).all()
+    assert np.array_equal(fg[m], fo[m]), "g max diff %g" % np.abs(fg[m] - fo[m]).max()
+    rg, jg, _ = g.get_moments()
+    ro, jo, _ = o.get_moments()
+    assert np.array_equal(rg[np.repeat(fl, C)], ro[np.repeat(fl, C)])
+    assert np.array_equal(jg[np.repeat(fl, C * D)], jo[np.repeat(fl, C * D)])
+
+
+@pytest.mark.parametrize("name,kw,frac,steps", CASES, ids=[c[0] for c in CASES])
+def test_species_fp64_bit_exact_vs_oracle(name, kw, frac, steps):
+    d, walls, rho, u = build(kw, frac)
+    print("case", name, flush=True)
+    assert_same(run(txg.Domain, d, walls, rho, u, steps), run(oracle.Oracle, d, walls, rho, u, steps), d, walls)
+
+
+@pytest.mark.parametrize("name,kw,frac,steps", [CASES[0], CASES[3]], ids=[CASES[0][0], CASES[3][0]])
+def test_species_exports_between_steps_and_restart(name, kw, frac, steps):
+    """Reading the state between steps leaves the march unchanged, and a
+    handle restarted from exported populations continues the same run."""
+    d, walls, rho, u = build(kw, frac)
+    a = run(txg.Domain, d, walls, rho, u, 0)
+    for k in (3, 1, 6):
+        a.step(k)
+        a.get_fi()
+        a.get_moments()
+    mid = a.get_fi()
+    b = run(txg.Domain, d, walls, rho, u, 0, fi=mid)
+    a.step(7)
+    b.step(7)
+    o = run(oracle.Oracle, d, walls, rho, u, 17)
+    assert_same(a, o, d, walls)
+    assert_same(b, o, d, walls)
+
+
+@pytest.mark.parametrize("name,kw,frac,steps", [CASES[0], CASES[3]], ids=[CASES[0][0], CASES[3][0]])
+def test_species_fp32_close_and_conservative(name, kw, frac, steps):
+    d64, walls, rho, u = build(kw, frac)
+    d32, _, _, _ = build(kw, frac, precision=txg.FP32)
+    g = run(txg.Domain, d32, walls, rho, u, 0)
+    m0 = g.get_moments()[0].reshape(-1, d32.ncomponents).sum(0)
+    g.step(steps)
+    o = run(oracle.Oracle, d64, walls, rho, u, steps)
+    fl = np.repeat(walls == 0, d32.Q * d32.ncomponents)
+    fg, fo = g.get_fi()[fl], o.get_fi()[fl]
+    assert np.abs(fg - fo).max() <= 1e-5 * np.abs(fo).max()
+    m1 = g.get_moments()[0].reshape(-1, d32.ncomponents).sum(0)
+    assert np.allclose(m1, m0, rtol=1e-6, atol=0)
+
+
+def test_species_refuse_flow_calls():
+    d, walls, rho, u = build(CASES[2][1], 0.0)
+    dom = run(txg.Domain, d, walls, rho, u, 1)
+    for call in (dom.diagnostics, dom.delta_norm, dom.flow_sums, dom.reduce, lambda: dom.tune("onepass")):
+        with pytest.raises(txg.TxgError):
+            call()
+    bt = txg.Batch()
+    with pytest.raises(txg.TxgError):
+        bt.add(dom)
+    bt.close()
+    with pytest.raises(txg.TxgError, match="forces"):
+        txg.Domain(txg.make_desc(transport=1, ncomponents=2, g=[[0, 0.1], [0.1, 0]]))
+    with pytest.raises(txg.TxgError, match="1 or 2 components"):
+        txg.Domain(txg.make_desc(ncomponents=3, isotropy_order=0))
