@@ -153,8 +153,13 @@ def main():
     achieved = ba * n / (coll_ms * 1e-3) / 1e9
     step_gbs = ba * n / ((coll_ms + dens_ms) * 1e-3) / 1e9
     traffic = traffic_for(args.workload)
-    masses = dom.reduce()[: d.ncomponents]
-    shape = dom.tune_info()
+    if d.transport:
+        # species distributions have no flow sums: their masses from the export
+        masses = dom.get_moments()[0].reshape(-1, d.ncomponents).sum(axis=0)
+        shape = None
+    else:
+        masses = dom.reduce()[: d.ncomponents]
+        shape = dom.tune_info()
     onepass = prof["density"][1] == 0
     out = {
         "metric": "MLUPS",
@@ -173,12 +178,13 @@ def main():
                    "nodes": n, "components": d.ncomponents,
                    "collision": "MRT" if d.relaxation_mode == txg.MRT else "BGK",
                    "parallelism": "single GPU",
-                   "step_kernel": "one-pass" if onepass else "two-pass",
+                   "step_kernel": "species" if d.transport else ("one-pass" if onepass else "two-pass"),
                    "tile": [shape[0], shape[1]] if shape and onepass else None,
                    "zseg": shape[2] if shape and onepass else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "one-pass step" if onepass else "collide pass", "bytes_per_node_step": ba,
+                     "kernel": "species step" if d.transport else ("one-pass step" if onepass else "collide pass"),
+                     "bytes_per_node_step": ba,
                      "collide_ms": round(coll_ms, 5), "density_ms": round(dens_ms, 5),
                      "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "mass": [float(x) for x in masses],

@@ -179,6 +179,9 @@ WORKLOADS = {
     "c2_drainage256_fp16": "c2_drainage256 with FP32 compute and shifted-FP16 storage",
     "c2_drainage256_fp64": "c2_drainage256 in FP64",
     "c2_drainage64_fp32": "c2 recipe at 64^3, FP32",
+    "t1_species128_fp32": "D3Q19 two passive-scalar species 128^3 through the c2 sphere pack (porosity 0.40), "
+                          "tau 0.8 / 1.2, a sphere of species A in species B, FP32",
+    "t1_species128_fp64": "t1_species128 in FP64",
 }
 
 
@@ -196,6 +199,13 @@ def workload(name):
         d = drainage_desc(n, precision=prec)
         walls, rho, u = drainage_ic(n, seed=2024)
         return d, walls, rho, u, WORKLOADS[name]
+    if name.startswith("t1_species"):
+        prec = {"fp32": txg.FP32, "fp64": txg.FP64}[name.split("_")[2]]
+        d = txg.make_desc(ndims=3, ncomponents=2, shape=(128, 128, 128), periodic=(1, 1, 1), isotropy_order=0,
+                          tau=[0.8, 1.2], precision=prec, transport=1)
+        walls = sphere_pack((128, 128, 128), 0.40, 6, 12, seed=7)
+        psi = ball_ic((128, 128, 128), 24, (1.0, 0.0), (0.0, 1.0)) * np.repeat(walls == 0, 2)
+        return d, walls, psi, np.zeros(d.nnodes * 6), WORKLOADS[name]
     if name == "c0_bubble200_fp64":
         d = bubble_desc(2, 200, precision=txg.FP64)
         walls, rho, u = bubble_ic(2, 200, 40)
