@@ -6,11 +6,15 @@
 Prints one JSON line.  `value` is MLUPS (lattice nodes x steps / s / 1e6,
 every node counted) of the whole job with the state resident in HBM; one
 step is one timestep of the reference's flow loop (LBMRun2) over the whole
-lattice.  `roofline` prices the dominant kernel (the collide pass) at the
-algorithmic bytes per node-step B_alg = 2*Q*C*s_f + 1 (populations read
-once and written once, plus a 1-byte node tag) against MI355X's 8 TB/s,
-with durations from HIP events on the domain's stream over the timed
-region.  `cpu_baseline` times the FP64 CPU oracle (this repository's
+lattice; `fluid_mlups` counts pore nodes only.  `roofline` prices the
+dominant kernel (the one-pass step kernel, or the collide pass of the
+two-pass step, or the species step) at the algorithmic bytes per node-step
+B_alg = 2*Q*C*s_f + 1 (populations read once and written once, plus a
+1-byte node tag; the rest population one format wider with FP32/FP16
+storage) against MI355X's 8 TB/s, with durations from HIP events on the
+domain's stream over the timed region; `copy_gbs` is a plain device copy
+timed in the same run (the practical ceiling) and `copy_frac` the kernel's
+rate against it.  `cpu_baseline` times the FP64 CPU oracle (this repository's
 restatement of the reference step; the reference itself needs PETSc 3.6,
 which the GPU machine lacks) on a bounded sample of the same workload.
 """
@@ -82,8 +86,9 @@ def cpu_baseline(name, seconds):
             "sample": "%s, %d steps after 1 warm-up step, FP64 oracle (C, OpenMP)" % (name, steps)}
 
 
-def copy_probe(mib):
-    """Time a device-to-device copy of mib MiB (torch, 20 repetitions)."""
+def copy_gbs(mib):
+    """Read + write GB/s of a device-to-device copy of mib MiB (torch, 20
+    repetitions): the practical bandwidth ceiling of a streaming kernel."""
     import torch
     n = mib * (1 << 20) // 4
     a = torch.ones(n, device="cuda")
@@ -97,7 +102,14 @@ def copy_probe(mib):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / 20
-    print(json.dumps({"copy_MiB": mib, "ms": round(ms, 4), "GB/s_read_plus_write": round(2 * n * 4 / ms / 1e6, 1)}))
+    del a, b
+    torch.cuda.empty_cache()
+    return ms, 2 * n * 4 / ms / 1e6
+
+
+def copy_probe(mib):
+    ms, gbs = copy_gbs(mib)
+    print(json.dumps({"copy_MiB": mib, "ms": round(ms, 4), "GB/s_read_plus_write": round(gbs, 1)}))
 
 
 def main():
@@ -147,12 +159,14 @@ def main():
     n = d.nnodes
     ms_step = wall / args.steps * 1e3
     mlups = n * args.steps / wall / 1e6
+    fluid = int((np.asarray(walls) == 0).sum())
     ba = b_alg(d)
     coll_ms = prof["collide"][0] / max(1, prof["collide"][1])
     dens_ms = prof["density"][0] / max(1, prof["density"][1])
     achieved = ba * n / (coll_ms * 1e-3) / 1e9
     step_gbs = ba * n / ((coll_ms + dens_ms) * 1e-3) / 1e9
     traffic = traffic_for(args.workload)
+    copy_bw = copy_gbs(256)[1]  # the same GPU's plain-copy bandwidth, measured now
     if d.transport:
         # species distributions have no flow sums: their masses from the export
         masses = dom.get_moments()[0].reshape(-1, d.ncomponents).sum(axis=0)
@@ -186,7 +200,9 @@ def main():
                      "kernel": "species step" if d.transport else ("one-pass step" if onepass else "collide pass"),
                      "bytes_per_node_step": ba,
                      "collide_ms": round(coll_ms, 5), "density_ms": round(dens_ms, 5),
-                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                     "copy_gbs": round(copy_bw, 1), "copy_frac": round(achieved / copy_bw, 4)},
+        "fluid_mlups": round(fluid * args.steps / wall / 1e6, 1),
         "mass": [float(x) for x in masses],
     }
     if args.cpu_seconds > 0:
