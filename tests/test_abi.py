@@ -66,7 +66,11 @@ def test_create_rejects_bad_descriptors_before_touching_the_device():
     """Validation errors are reported with a message and no handle."""
     for kw, msg in ((dict(ndims=4), "ndims"), (dict(ncomponents=0), "ncomponents"),
                     (dict(isotropy_order=6), "isotropy"), (dict(ncomponents=3), "components"),
-                    (dict(tau=[0.4]), "tau")):
+                    (dict(tau=[0.4]), "tau"), (dict(shape=(2, 8, 8)), "at least 3"),
+                    (dict(shape=(1400, 1400, 1400)), "too large"),
+                    (dict(transport=1, gvt=[1e-5, 0, 0]), "body"),
+                    (dict(transport=1, periodic=(0, 1, 1), bc={"xm": [1.0]}), "boundary"),
+                    (dict(transport=2), "transport")):
         d = txg.make_desc(**{k: v for k, v in kw.items() if k not in ("ndims", "ncomponents")},
                           ncomponents=kw.get("ncomponents", 1))
         if "ndims" in kw:
