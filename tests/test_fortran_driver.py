@@ -58,9 +58,8 @@ def test_species_through_fortran_driver(tmp_path):
 
     out = tmp_path / "test_solution"
     out.mkdir()
-    args = ["-transport_discretization", "d2q9", "-nspecies", "2", "-specie1_name", "tracer",
-            "-tau_tracer", "0.8", "-tau_specie2", "1.3", "-psi_inner", "1.0,0.2", "-psi_outer", "0.1,0.0"]
-    r = subprocess.run([EXE, OPTS, "-output_file_prefix", str(out) + "/"] + args, cwd=tmp_path,
+    opts = os.path.join(os.path.dirname(OPTS), "bubble_2D_species.opts")
+    r = subprocess.run([EXE, opts, "-output_file_prefix", str(out) + "/"], cwd=tmp_path,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     d = txg.make_desc(ndims=2, ncomponents=2, shape=(128, 128), periodic=(1, 1), isotropy_order=0,
