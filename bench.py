@@ -87,24 +87,42 @@ def cpu_baseline(name, seconds):
 
 
 def copy_gbs(mib):
-    """Read + write GB/s of a device-to-device copy of mib MiB (torch, 20
-    repetitions): the practical bandwidth ceiling of a streaming kernel."""
-    import torch
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, device="cuda")
-    b = torch.empty_like(a)
+    """Read + write GB/s of a device-to-device copy of mib MiB (hipMemcpyAsync,
+    20 repetitions between HIP events): the practical bandwidth ceiling of a
+    streaming kernel.  Driven through the HIP runtime the step library links
+    (one runtime per process: torch bundles another)."""
+    import ctypes as C
+    txg.lib()
+    hip = C.CDLL("libamdhip64.so.7")
+    nbytes = mib * (1 << 20)
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed: %d" % (what, rc))
+
+    a, b = C.c_void_p(), C.c_void_p()
+    ok(hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)), "hipMalloc")
+    ok(hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)), "hipMalloc")
+    ok(hip.hipMemset(a, 1, C.c_size_t(nbytes)), "hipMemset")
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ok(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+    ok(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+    kind = 3  # hipMemcpyDeviceToDevice
     for _ in range(3):
-        b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
+        ok(hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), kind, None), "hipMemcpyAsync")
+    ok(hip.hipEventRecord(e0, None), "hipEventRecord")
     for _ in range(20):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / 20
-    del a, b
-    torch.cuda.empty_cache()
-    return ms, 2 * n * 4 / ms / 1e6
+        ok(hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), kind, None), "hipMemcpyAsync")
+    ok(hip.hipEventRecord(e1, None), "hipEventRecord")
+    ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+    ms = C.c_float()
+    ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+    for e in (e0, e1):
+        hip.hipEventDestroy(e)
+    for x in (a, b):
+        hip.hipFree(x)
+    t = ms.value / 20
+    return t, 2 * nbytes / t / 1e6
 
 
 def copy_probe(mib):

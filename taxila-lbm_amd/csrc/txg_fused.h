@@ -370,6 +370,9 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
   // phase A for plane zz: own column into f (kept for phase B), ring
   // columns; mo and mr are the plane's masks (loaded one step earlier)
   auto phase_a = [&](int zz, R (&f)[C][Q], R (&d0)[C], uint32_t mo, const RingMasks& mr) {
+    // loads first: a wave in phase A issues ahead of waves computing phase B
+    // (of the other workgroup on the CU), so memory and arithmetic overlap
+    __builtin_amdgcn_s_setprio(1);
     const int slot = slot_of(zz);
     uint32_t zr[3];
     zrows(zz, zr);
@@ -386,6 +389,7 @@ __device__ __forceinline__ void fused3_item(const Params<R>& p, const Bufs<S, R>
 #pragma unroll
       for (int m = 0; m < C; m++) srho[(slot * C + m) * NCOL + rcol[k]] = rr[m];
     }
+    __builtin_amdgcn_s_setprio(0);
   };
   auto phase_a_now = [&](int zz, R (&f)[C][Q], R (&d0)[C]) -> uint32_t {
     const uint32_t mo = own_mask(zz);
