@@ -164,6 +164,14 @@ def run(scale, target, steps, g, precision=txg.FP32):
         COEX[g] = coexistence(g)
     major, minor = COEX[g]
     gw = tuple(scale * v for v in workloads.wall_adhesion(target, g=g, rho_major=major, rho_minor=minor))
+    return droplet_angle(gw, steps, g, precision)
+
+
+def droplet_angle(gw, steps, g, precision=txg.FP32):
+    """(gw, angle, per-slice angles) of a droplet on a wall with these -gw."""
+    if g not in COEX:
+        COEX[g] = coexistence(g)
+    major, minor = COEX[g]
     d = droplet_desc(gw, precision, g=g)
     walls, rho, u = droplet_ic(major=major, minor=minor)
     dom = txg.Domain(d)

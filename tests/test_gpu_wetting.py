@@ -1,0 +1,23 @@
+"""Wettability on the HIP path: sessile droplets reach the contact angles
+measured when the -gw values were calibrated (GPU).
+
+The fluid-solid force (LBMAddFluidSolidForcesD3, lbm_forcing.F90:1326-1373)
+is pinned bit for bit by the parity tests; this checks what it is for: with
+the c2 fluid physics at g = 0.15 a neutral wall (gw = 0) holds a droplet at
+90 degrees, and the calibrated pairs hold 45 and 120 degrees
+(taxila-lbm_amd/contact_angle.py, 96 x 96 x 56, 8,000 steps).
+"""
+import pytest
+
+import contact_angle as ca
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("gw,theta,tol", [((0.0, 0.0), 90.0, 4.0),
+                                          ((-0.0521, 0.0521), 45.0, 2.0),
+                                          ((0.0481, -0.0481), 120.0, 2.0)])
+def test_droplet_contact_angle(gw, theta, tol):
+    _, got, parts = ca.droplet_angle(gw, 8000, 0.15)
+    print("gw", gw, "angle", got, parts, flush=True)
+    assert abs(got - theta) <= tol
