@@ -192,6 +192,10 @@ struct txg_handle {
   Params<double> p64{};
   Params<float> p32{};
   std::vector<uint8_t> hsolid;  // wall nodes, for the exports' zeroing
+  // two steps captured as a HIP graph (launch-bound domains), one per
+  // starting orientation, with the launch arguments they were captured with
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<unsigned char> gkey[2];
   // event timing of the step passes (txg_profile_enable)
   int prof = 0;
   int ev_next = 0;
@@ -1067,67 +1071,154 @@ int autotune(txg_handle* h, int reps) {
 }
 }  // namespace
 
-int txg_step(txg_handle* h, int32_t nsteps) {
-  if (!h || nsteps < 0) return TXG_ERR_ARG;
-  if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
+namespace {
+// one step of the handle, queued on its stream: the species step, the
+// one-pass step, or the two passes (collide, then the density of the new
+// state); the autotuner runs before the first one-pass step
+int eager_step(txg_handle* h, bool fused) {
+  int rc;
   if (h->d.transport) {
     // the transport phases of LBMRun2: collision, stream, wall bounce-back, moments
-    for (int s = 0; s < nsteps; s++) {
-      AnyBufs b = bufs(h, h->cur);
-      int rc;
-      {
-        ProfScope ps(h, 0);
-        rc = h->hip(h->ks.species(h->params(), b.ptr(), h->local_state ? 0 : 1, nullptr, h->N, h->stream),
-                    "species kernel");
-      }
-      if (rc) return rc;
-      h->cur = 1 - h->cur;
-      h->local_state = 0;
-      h->rho_valid = 0;
-    }
-    return TXG_OK;
-  }
-  if (h->batch && nsteps > 0) batch_mark_dirty(h->batch);  // the batch's orientation snapshot goes stale
-  const bool fused = use_fused(h);
-  for (int s = 0; s < nsteps; s++) {
-    int rc;
-    if (fused && !h->local_state && !h->tuned) {
-      rc = autotune(h, 3);
-      if (rc) return rc;
-    }
-    if (fused && !h->local_state) {
-      const FusedGeom g = fused_geom(h);
-      // one pass: pull, density (LDS), forces, collide
-      AnyBufs b = bufs(h, h->cur);
-      {
-        ProfScope ps(h, 0);
-        rc = h->hip(h->ks.fused[h->tile](h->params(), b.ptr(), &g, h->stream), "one-pass kernel");
-      }
-      if (rc) return rc;
-      h->cur = 1 - h->cur;
-      h->rho_valid = 0;
-      continue;
-    }
-    // two passes: collide (from the local initial state, or pulled), then
-    // the density of the new state for the next collide
     AnyBufs b = bufs(h, h->cur);
-    int pulled = h->local_state ? 0 : 1;
     {
       ProfScope ps(h, 0);
-      rc = h->hip(h->ks.collide(h->params(), b.ptr(), pulled, pulled, nullptr, h->N, h->stream), "collide kernel");
+      rc = h->hip(h->ks.species(h->params(), b.ptr(), h->local_state ? 0 : 1, nullptr, h->N, h->stream),
+                  "species kernel");
     }
     if (rc) return rc;
     h->cur = 1 - h->cur;
     h->local_state = 0;
     h->rho_valid = 0;
-    if (fused) continue;
-    AnyBufs b2 = bufs(h, h->cur);
+    return TXG_OK;
+  }
+  if (fused && !h->local_state && !h->tuned) {
+    rc = autotune(h, 3);
+    if (rc) return rc;
+  }
+  if (fused && !h->local_state) {
+    const FusedGeom g = fused_geom(h);
+    // one pass: pull, density (LDS), forces, collide
+    AnyBufs b = bufs(h, h->cur);
     {
-      ProfScope ps(h, 1);
-      rc = h->hip(h->ks.density(h->params(), b2.ptr(), 1, 1, h->N, h->stream), "density kernel");
+      ProfScope ps(h, 0);
+      rc = h->hip(h->ks.fused[h->tile](h->params(), b.ptr(), &g, h->stream), "one-pass kernel");
     }
     if (rc) return rc;
-    h->rho_valid = 1;
+    h->cur = 1 - h->cur;
+    h->rho_valid = 0;
+    return TXG_OK;
+  }
+  // two passes: collide (from the local initial state, or pulled), then
+  // the density of the new state for the next collide
+  AnyBufs b = bufs(h, h->cur);
+  int pulled = h->local_state ? 0 : 1;
+  {
+    ProfScope ps(h, 0);
+    rc = h->hip(h->ks.collide(h->params(), b.ptr(), pulled, pulled, nullptr, h->N, h->stream), "collide kernel");
+  }
+  if (rc) return rc;
+  h->cur = 1 - h->cur;
+  h->local_state = 0;
+  h->rho_valid = 0;
+  if (fused) return TXG_OK;
+  AnyBufs b2 = bufs(h, h->cur);
+  {
+    ProfScope ps(h, 1);
+    rc = h->hip(h->ks.density(h->params(), b2.ptr(), 1, 1, h->N, h->stream), "density kernel");
+  }
+  if (rc) return rc;
+  h->rho_valid = 1;
+  return TXG_OK;
+}
+
+// Domains up to this many nodes step through a captured graph of two steps
+// (their kernels last a few microseconds, comparable to a launch).
+constexpr long long kGraphMaxNodes = 1 << 20;
+
+// everything the two captured steps bake in: the kernel parameters and both
+// buffer orientations, the kernels chosen, the one-pass launch shape and the
+// orientation the capture starts from
+std::vector<unsigned char> step_key(txg_handle* h, bool fused) {
+  std::vector<unsigned char> k;
+  auto put = [&](const void* p, size_t n) {
+    const unsigned char* c = (const unsigned char*)p;
+    k.insert(k.end(), c, c + n);
+  };
+  put(h->params(), h->prec == TXG_FP64 ? sizeof(Params<double>) : sizeof(Params<float>));
+  const size_t bs = h->prec == TXG_FP64 ? sizeof(Bufs<double, double>)
+                                        : (h->prec == TXG_FP32 ? sizeof(Bufs<float, float>) : sizeof(Bufs<__half, float>));
+  for (int in = 0; in < 2; in++) {
+    AnyBufs b = bufs(h, in);
+    put(b.ptr(), bs);
+  }
+  const void* fns[4] = {(const void*)h->ks.species, (const void*)h->ks.collide, (const void*)h->ks.density,
+                        fused ? (const void*)h->ks.fused[h->tile] : nullptr};
+  put(fns, sizeof(fns));
+  const int meta[4] = {h->cur, h->tile, fused ? 1 : 0, h->d.transport};
+  put(meta, sizeof(meta));
+  if (fused) {
+    const FusedGeom g = fused_geom(h);
+    const int gi[6] = {g.ntx, g.nty, g.ntiles, g.nseg, g.zseg, g.nblocks};
+    put(gi, sizeof(gi));
+  }
+  return k;
+}
+
+// two steps by replaying the captured graph, capturing it first when the
+// launch arguments changed since the last capture
+int graph_two_steps(txg_handle* h, bool fused) {
+  std::vector<unsigned char> key = step_key(h, fused);
+  hipGraphExec_t& ge = h->gexec[h->cur];
+  if (!ge || key != h->gkey[h->cur]) {
+    if (ge) {
+      (void)hipGraphExecDestroy(ge);
+      ge = nullptr;
+    }
+    int rc = h->hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+    if (rc) return rc;
+    const int cur0 = h->cur, rv0 = h->rho_valid;
+    int rc1 = eager_step(h, fused);
+    if (!rc1) rc1 = eager_step(h, fused);
+    hipGraph_t g = nullptr;
+    rc = h->hip(hipStreamEndCapture(h->stream, &g), "graph capture");
+    // the captured steps have not run: the host state stays where it was
+    h->cur = cur0;
+    h->rho_valid = rv0;
+    if (rc1 || rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc1 ? rc1 : rc;
+    }
+    rc = h->hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "graph instantiate");
+    (void)hipGraphDestroy(g);
+    if (rc) {
+      ge = nullptr;
+      return rc;
+    }
+    h->gkey[h->cur] = std::move(key);
+  }
+  int rc = h->hip(hipGraphLaunch(ge, h->stream), "graph launch");
+  if (rc) return rc;
+  // two steps flip the orientation twice; the last pass leaves rho[] as an
+  // eager second step would
+  h->local_state = 0;
+  h->rho_valid = (!h->d.transport && !fused) ? 1 : 0;
+  return TXG_OK;
+}
+}  // namespace
+
+int txg_step(txg_handle* h, int32_t nsteps) {
+  if (!h || nsteps < 0) return TXG_ERR_ARG;
+  if (!h->initialized) return h->fail(TXG_ERR_STATE, "txg_init has not been called");
+  if (h->batch && nsteps > 0) batch_mark_dirty(h->batch);  // the batch's orientation snapshot goes stale
+  const bool fused = !h->d.transport && use_fused(h);
+  int s = 0;
+  while (s < nsteps) {
+    // steady stepping of a small domain, not profiled: two steps per graph
+    const bool graph = !h->prof && !h->local_state && (!fused || h->tuned) && (long long)h->N <= kGraphMaxNodes &&
+                       nsteps - s >= 2;
+    const int rc = graph ? graph_two_steps(h, fused) : eager_step(h, fused);
+    if (rc) return rc;
+    s += graph ? 2 : 1;
   }
   return TXG_OK;
 }
@@ -1365,6 +1456,8 @@ int txg_destroy(txg_handle* h) {
     (void)hipStreamSynchronize(h->stream);
   }
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  for (hipGraphExec_t e : h->gexec)
+    if (e) (void)hipGraphExecDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return TXG_OK;

@@ -1,0 +1,63 @@
+"""Steady stepping of small domains replays a captured HIP graph of two
+steps (txg_step); a profiled handle steps eagerly, one launch per pass.
+The graph must march the same state as the eager launches and the FP64
+oracle, replay after replay, with the program's own eager work (exports,
+odd single steps, syncs) between the replays (GPU)."""
+import numpy as np
+import pytest
+
+import cases
+import oracle
+import txg
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("d2_two_pass", dict(ndims=2, ncomponents=2, shape=(40, 36), periodic=(1, 0), isotropy_order=8,
+                         g=[[0, 0.1], [0.1, 0]], tau=[1.0, 0.7], gw=[[0.03, -0.03]], gvt=[1e-5, 0.0])),
+    ("d3_one_pass", dict(ndims=3, ncomponents=2, shape=(24, 20, 18), periodic=(1, 1, 0), isotropy_order=4,
+                         g=[[0, 0.1], [0.1, 0]], tau=[1.0, 1.0])),
+    ("d3_one_pass_bc", dict(ndims=3, ncomponents=2, shape=(22, 18, 14), periodic=(0, 0, 1), isotropy_order=4,
+                            g=[[0, 0.1], [0.1, 0]], bc={"xm": (txg.BC_MOMENTUM, [0.01, 0, 0, 0, 0, 0]),
+                                                        "xp": (txg.BC_DENSITY, [0.9, 0.1])})),
+    ("d3_species", dict(ndims=3, ncomponents=2, shape=(20, 18, 16), periodic=(1, 0, 1), isotropy_order=0,
+                        tau=[0.8, 1.3], transport=1)),
+]
+
+
+def build(kw):
+    d = txg.make_desc(**kw)
+    shape = (d.nx, d.ny, d.nz) if d.ndims == 3 else (d.nx, d.ny)
+    walls = cases.random_walls(shape, 0.1, seed=5)
+    rng = np.random.default_rng(9)
+    rho = (0.1 + rng.random(d.nnodes * d.ncomponents)) * np.repeat(walls == 0, d.ncomponents)
+    u = 0.002 * rng.standard_normal(d.nnodes * d.ndims * d.ncomponents)
+    return d, walls, rho, u
+
+
+def start(cls, d, walls, rho, u, profile=False):
+    dom = cls(d)
+    dom.set_walls(walls)
+    dom.set_state(rho, u)
+    dom.init()
+    if profile:
+        dom.profile(True)
+    return dom
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_graph_replays_match_eager_and_oracle(name, kw):
+    d, walls, rho, u = build(kw)
+    g = start(txg.Domain, d, walls, rho, u)
+    e = start(txg.Domain, d, walls, rho, u, profile=True)
+    o = start(oracle.Oracle, d, walls, rho, u)
+    fl = np.repeat(walls == 0, d.Q * d.ncomponents)
+    total = 0
+    for k in (1, 6, 3, 2, 9, 4, 1, 8):
+        for dom in (g, e, o):
+            dom.step(k)
+        total += k
+        fg, fe, fo = g.get_fi(), e.get_fi(), o.get_fi()
+        assert np.array_equal(fg[fl], fe[fl]), "graph != eager after %d steps" % total
+        assert np.array_equal(fg[fl], fo[fl]), "graph != oracle after %d steps" % total
+        g.sync()
