@@ -179,10 +179,8 @@ int txg_init(txg_handle* h);
 /* Replaces nsteps iterations of LBMRun2's flow body (lbm.F90:279-406):
  * FlowCollision, DistributionCommunicateFi, FlowStream, FlowBounceback,
  * FlowApplyBCs, FlowUpdateFlux.  Asynchronous on the handle's stream.
- * Domains of at most 2^20 nodes step two steps at a time through a HIP
- * graph captured on the handle's stream (not while profiling); the first
- * such call after a change of parameters captures it, so a caller must not
- * be capturing the handle's stream itself. */
+ * (A path that steps small domains through captured HIP graphs exists and
+ * is switched off in this build.) */
 int txg_step(txg_handle* h, int32_t nsteps);
 
 /* Copies the current state back in the layouts above.  fi is the

@@ -1132,8 +1132,11 @@ int eager_step(txg_handle* h, bool fused) {
 }
 
 // Domains up to this many nodes step through a captured graph of two steps
-// (their kernels last a few microseconds, comparable to a launch).
+// (their kernels last a few microseconds, comparable to a launch).  Switched
+// off: a variant of this path that also captured one-step graphs was
+// followed by a GPU memory fault whose cause has not been found.
 constexpr long long kGraphMaxNodes = 1 << 20;
+constexpr bool kUseGraphs = false;
 
 // everything the two captured steps bake in: the kernel parameters and both
 // buffer orientations, the kernels chosen, the one-pass launch shape and the
@@ -1171,6 +1174,10 @@ int graph_two_steps(txg_handle* h, bool fused) {
   hipGraphExec_t& ge = h->gexec[h->cur];
   if (!ge || key != h->gkey[h->cur]) {
     if (ge) {
+      // launches of the old graph may still be queued: let them finish
+      // before its kernel arguments go
+      int rc0 = h->hip(hipStreamSynchronize(h->stream), "graph sync");
+      if (rc0) return rc0;
       (void)hipGraphExecDestroy(ge);
       ge = nullptr;
     }
@@ -1214,8 +1221,8 @@ int txg_step(txg_handle* h, int32_t nsteps) {
   int s = 0;
   while (s < nsteps) {
     // steady stepping of a small domain, not profiled: two steps per graph
-    const bool graph = !h->prof && !h->local_state && (!fused || h->tuned) && (long long)h->N <= kGraphMaxNodes &&
-                       nsteps - s >= 2;
+    const bool graph = kUseGraphs && !h->prof && !h->local_state && (!fused || h->tuned) &&
+                       (long long)h->N <= kGraphMaxNodes && nsteps - s >= 2;
     const int rc = graph ? graph_two_steps(h, fused) : eager_step(h, fused);
     if (rc) return rc;
     s += graph ? 2 : 1;
