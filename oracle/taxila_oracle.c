@@ -8,7 +8,7 @@
  * 10 on D2Q9), per-mineral fluid-solid forces, body forces, bounce-back
  * (including ghost-999 faces and 900-902 specular tags), periodic faces and
  * the face boundary conditions: Dirichlet (density, pressure, pressure
- * inlet, one-component pressure outlet), Neumann (momentum, flux, flux
+ * inlet, pressure outlet of one or two components), Neumann (momentum, flux, flux
  * inlet/outlet), velocity and reflecting, with per-node face values.
  * psi = rho (no -flow_use_nonideal_eos).
  *
@@ -454,8 +454,8 @@ static const char* setup_bcs(orc_t* o) {
       return "pressure and flux BCs take one component";
     if ((flag == TXG_BC_PRESSURE_INLET || flag == TXG_BC_FLUX_INLET) && C > 2)
       return "pressure and flux inlets are implemented for 1 or 2 components";
-    if (flag == TXG_BC_PRESSURE_OUTLET && C != 1)
-      return "a pressure outlet of several components is not implemented";
+    if (flag == TXG_BC_PRESSURE_OUTLET && C > 2)
+      return "a pressure outlet is implemented for 1 or 2 components";
     if (flag == TXG_BC_REFLECTING && D == 2 && f == 0)
       return "a reflecting xm face in 2-D is not implemented";
     o->bckind[f] = t;
@@ -1271,6 +1271,40 @@ static void bc_apply(orc_t* o) {
 }
 
 /* BCUpdateRho_D2/D3 (lbm_bc.F90:436-611) */
+/* FlowUpdateBCPressureOutletD3/D2 (lbm_flow.F90:2019-2233), called first in
+ * FlowApplyBCs for two components (:1964-1972): each fluid node of a
+ * pressure-outlet face takes the densities of the face's pressure at the
+ * rho_1/rho_t of its inward neighbour (of itself when that is a wall), from
+ * rho as the previous step left it. */
+static void bc_pressure_outlet(orc_t* o) {
+  if (o->C != 2) return;
+  for (int f = 0; f < 2 * o->D; f++) {
+    if (o->d.bc_flags[f] != TXG_BC_PRESSURE_OUTLET) continue;
+    int a0[3], a1[3];
+    face_range(o, f, a0, a1);
+    const int dim = f / 2, step = (f % 2) ? -1 : 1;
+    for (int z = a0[2]; z <= a1[2]; z++)
+      for (int y = a0[1]; y <= a1[1]; y++)
+        for (int x = a0[0]; x <= a1[0]; x++) {
+          if (WALL(o, x, y, z) != 0) continue;
+          int c[3] = {x, y, z};
+          c[dim] += step;
+          if (WALL(o, c[0], c[1], c[2]) != 0) {
+            c[0] = x;
+            c[1] = y;
+            c[2] = z;
+          }
+          const double r1 = RHO(o, c[0], c[1], c[2], 0);
+          const double rho1frac = r1 / (r1 + RHO(o, c[0], c[1], c[2], 1));
+          double rho[MAXC] = {0.0, 0.0, 0.0, 0.0};
+          density_from_pressure(o, o->d.bc_data[f][0], rho1frac, rho);
+          const long i = face_index(o, f, x, y, z);
+          BCV(o, f, i, 0, 0) = rho[0];
+          BCV(o, f, i, 0, 1) = rho[1];
+        }
+  }
+}
+
 static void bc_update_rho(orc_t* o) {
   for (int f = 0; f < 2 * o->D; f++) {
     if (!node_bc_face(o, f)) continue;
@@ -1380,7 +1414,8 @@ int orc_step(orc_t* o, int nsteps) {
     bc_prestream(o);    /* FlowStream */
     stream(o);
     bounceback(o);      /* FlowBounceback */
-    calc_density(o);    /* FlowApplyBCs -> FlowCalcRhoForces */
+    bc_pressure_outlet(o); /* FlowApplyBCs: FlowUpdateBCPressureOutlet */
+    calc_density(o);    /* FlowCalcRhoForces */
     bc_dirichlet_to_rho(o);
     calc_forces(o);
     bc_apply(o);
@@ -1494,7 +1529,7 @@ int orc_phase(orc_t* o, int phase) {
     case 0: update_feq(o); collide(o); break;
     case 1: halo(o, o->fi, o->Q * o->C, 0); break;
     case 2: bc_prestream(o); stream(o); break;
-    case 3: bounceback(o); break;
+    case 3: bounceback(o); bc_pressure_outlet(o); break;
     case 4: calc_density(o); break;
     case 5: bc_dirichlet_to_rho(o); break;
     case 6: calc_forces(o); break;

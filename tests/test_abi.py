@@ -130,5 +130,8 @@ def test_unsupported_face_bcs_are_refused_before_touching_the_device():
         rc = txg.lib().txg_create(ctypes.byref(d), ctypes.byref(h))
         assert rc in (txg.ARG_ERROR, txg.UNSUPPORTED_ERROR) and not h.value
         import oracle
+        if kw["bc"].get("xm", (0,))[0] == B.BC_PRESSURE_OUTLET:
+            oracle.Oracle(d)  # the oracle has the two-component outlet; the HIP step does not yet
+            continue
         with pytest.raises(ValueError):
             oracle.Oracle(d)

@@ -281,3 +281,44 @@ def test_bc_values_per_node_follow_the_face_layout():
     assert np.allclose(j[sel, 0, 0], 1e-3 * (1 + y[sel] + 10 * z[sel]), rtol=1e-12)
     assert np.allclose(j[sel, 0, 2], -2e-4, rtol=1e-12)
     assert np.allclose(j[sel, 0, 1], 0.0, atol=1e-16)
+
+
+@pytest.mark.parametrize("face", ["xp", "ym"])
+def test_two_component_pressure_outlet_tracks_inward_fraction(face):
+    """FlowUpdateBCPressureOutlet (lbm_flow.F90:2019-2233): each step the
+    face densities solve p = rho_t/3 + c0 g21 rho_1 rho_2 at the rho_1/rho_t
+    the inward neighbour had after the previous step (the node's own when
+    the neighbour is a wall)."""
+    p_out, g12, g21 = 0.32, 0.09, 0.11
+    shape = (14, 12, 10)
+    f = txg.FACES.index(face)
+    d = txg.make_desc(ndims=3, ncomponents=2, shape=shape, periodic=(0, 0, 1), isotropy_order=4,
+                      g=[[0, g12], [g21, 0]], tau=[1.0, 0.8],
+                      bc={face: (txg.BC_PRESSURE_OUTLET, [p_out]), "xm": [0.9, 0.1]})
+    x, y, z = coords(d)
+    dim = f // 2
+    pos = (x, y, z)[dim]
+    hi = shape[dim] - 1
+    inward_pos = hi - 1 if f % 2 else 1
+    walls = np.zeros(d.nnodes)
+    walls[(pos == inward_pos) & ((x + y + z) % 5 == 0)] = 1.0  # some inward neighbours are walls
+    rng = np.random.default_rng(2)
+    a = 0.2 + 0.6 * rng.random(d.nnodes)
+    rho = np.stack([a, 1.0 - a], axis=1).reshape(-1) * np.repeat(walls == 0, 2)
+    o = oracle.Oracle(d)
+    o.set_walls(walls)
+    o.set_state(rho)
+    o.init()
+    for _ in range(3):
+        prev = o.get_moments()[0].reshape(d.nnodes, 2)
+        o.step(1)
+        now = o.get_moments()[0].reshape(d.nnodes, 2)
+        face_nodes = np.nonzero((pos == (hi if f % 2 else 0)) & (walls == 0) & ~on_face(d, 0))[0]
+        assert len(face_nodes) > 20
+        shift = (-1 if f % 2 else 1) * (1, d.nx, d.nx * d.ny)[dim]
+        for n in face_nodes:
+            src = n + shift if walls[n + shift] == 0 else n
+            frac = prev[src, 0] / (prev[src, 0] + prev[src, 1])
+            r1, r2 = now[n]
+            assert r1 / (r1 + r2) == pytest.approx(frac, rel=1e-12, abs=0)
+            assert (r1 + r2) / 3 + 6 * g21 * r1 * r2 == pytest.approx(p_out, rel=1e-12, abs=0)
